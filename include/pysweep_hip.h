@@ -1,0 +1,167 @@
+/*
+ * pysweep_hip.h -- C ABI of libpysweep_hip.so, the MI355X (gfx950) runtime
+ * under pysweep_amd.
+ *
+ * The reference drives its GPU through PyCUDA.  Each entry point below
+ * replaces one PyCUDA call site of the reference (cited as file:line into
+ * the reference tree) or one host loop of its solver.
+ *
+ * Conventions
+ *   - every entry returns int: 0 on success, nonzero on failure; the
+ *     message of the last failure on the calling thread is ps_last_error();
+ *   - no entry throws across the ABI;
+ *   - host buffers are caller-owned and only borrowed for the call;
+ *   - device buffers and modules are owned by the caller through the
+ *     handles returned here and released with ps_free / ps_module_unload;
+ *   - all device work is ordered on one HIP stream per process (created by
+ *     ps_init, default priority); calls are not thread-safe.
+ */
+#ifndef PYSWEEP_HIP_H
+#define PYSWEEP_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void *ps_module;   /* loaded code object (hipModule_t)   */
+typedef void *ps_function; /* kernel handle (hipFunction_t)       */
+typedef void *ps_event;    /* timing event (hipEvent_t)           */
+typedef void *ps_graph;    /* instantiated graph (hipGraphExec_t) */
+
+/* ABI version of this header. */
+int ps_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char *ps_last_error(void);
+
+/* Number of visible GPUs; 0 without a GPU (never an error). */
+int ps_device_count(int *count);
+
+/* Select the device and create the work stream.
+ * Replaces cuda.init(); cuda.Device(rank).make_context() (core/block.py:36-38). */
+int ps_init(int device);
+
+/* Release the stream.  Replaces cuda_context.pop() (core/process.py:31-32). */
+int ps_shutdown(void);
+
+/* Compile HIP source text for gfx950 with hiprtc; no device is needed.
+ * On success *code (size *size) is a code object the caller frees with
+ * ps_free_host; *log (may be NULL on success) is the compiler log, also
+ * freed with ps_free_host.  Replaces SourceModule(...) in
+ * io.buildGPUSource (core/io.py:146-155). */
+int ps_compile(const char *source, const char *name, const char *const *options, int noptions,
+               char **code, size_t *size, char **log);
+void ps_free_host(void *p);
+
+/* Load / unload a code object on the current device. */
+int ps_module_load(const void *code, size_t size, ps_module *module);
+int ps_module_unload(ps_module module);
+
+/* Device address and size of a __device__ / __constant__ global.
+ * Replaces source_mod.get_global(name) (core/io.py:178, equations/heat.py:43). */
+int ps_module_get_global(ps_module module, const char *name, void **dptr, size_t *bytes);
+
+/* Kernel handle.  Replaces source_mod.get_function(name) (core/block.py:92-96, 213). */
+int ps_module_get_function(ps_module module, const char *name, ps_function *function);
+
+/* Registers, LDS bytes, max threads per block of a kernel (for the autotuner). */
+int ps_function_info(ps_function function, int *vgprs_or_regs, int *lds_bytes, int *max_threads);
+
+/* Device memory.  Replaces cuda.mem_alloc (core/block.py:99-104). */
+int ps_malloc(void **dptr, size_t bytes);
+int ps_free(void *dptr);
+
+/* Copies (synchronous with respect to the host).
+ * Replace cuda.memcpy_htod / memcpy_dtoh (core/functions.py:22, 32; core/io.py:184). */
+int ps_memcpy_htod(void *dst, const void *src, size_t bytes);
+int ps_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+int ps_memcpy_dtod(void *dst, const void *src, size_t bytes);
+/* Fill bytes (stream ordered). */
+int ps_memset(void *dst, int value, size_t bytes);
+
+/* Launch a kernel on the work stream.  args[i] points at the i-th kernel
+ * argument.  Replaces the PyCUDA Function call in Geometry.callGPU /
+ * callStandardGPU (core/geometry.py:38-46). */
+int ps_launch(ps_function function, unsigned gx, unsigned gy, unsigned gz, unsigned bx, unsigned by,
+              unsigned bz, unsigned shared_bytes, void **args);
+
+/* Wait for the work stream.  Replaces cuda.Context.synchronize()
+ * (core/functions.py:31, 60, 84, 142). */
+int ps_synchronize(void);
+
+/* Timing events on the work stream. */
+int ps_event_create(ps_event *event);
+int ps_event_record(ps_event event);
+int ps_event_elapsed_ms(ps_event start, ps_event stop, float *ms);
+int ps_event_destroy(ps_event event);
+
+/* ---------------------------------------------------------------------- */
+/* Native phase sequencing.                                                */
+/* ---------------------------------------------------------------------- */
+
+/* Geometry argument every phase / standard kernel receives by value. */
+typedef struct ps_grid {
+    int nx, ny;   /* periodic domain (rows x, columns y; y contiguous)   */
+    int T;        /* ring levels: absolute level A lives in slot A % T    */
+    int xoff;     /* x lattice offset of this cycle (0 or b/2)            */
+    int tiles_x;  /* tile rows                                            */
+    int tiles_y;  /* tile columns                                         */
+} ps_grid;
+
+/* One swept solve on a device-resident ring state[T][nv][nx][ny].
+ * Phase kernels have the reference's names and leading arguments
+ * K(double *state, int globalTimeStep, int sweptStep, ps_grid g)
+ * (core/pysweep.cu:64-249), sweptStep being the absolute level read by the
+ * first sub-step. */
+typedef struct ps_swept_plan {
+    void *state;
+    int nx, ny;
+    int b;          /* block size                           */
+    int ops;        /* operating points k                   */
+    int its;        /* intermediate steps s                 */
+    int T;          /* 2*P + s                              */
+    int threads;    /* threads per workgroup                */
+    ps_function up, ybridge, xbridge, octahedron, down;
+} ps_swept_plan;
+
+/* Run cycles [c_begin, c_end) of the swept schedule with G UpPrism cycles:
+ * cycle 0 = FirstPrism (UpPyramid + YBridge), 1..G = UpPrism (XBridge,
+ * Octahedron, YBridge), G+1 = LastPrism (XBridge + DownPyramid).
+ * Replaces Solver.sweptSolve and functions.FirstPrism/UpPrism/LastPrism
+ * (core/solver.py:158-176; core/functions.py:12-88); the window roll and
+ * shift (functions.py:90-123, io.py:134-144) become the lattice offset and
+ * ring indexing. */
+int ps_swept_run(const ps_swept_plan *plan, int G, int c_begin, int c_end);
+
+/* Standard decomposition: one launch per stage. */
+typedef struct ps_standard_plan {
+    void *state;
+    int nx, ny;
+    int its;
+    int T;          /* its + 1                              */
+    int bx, by;     /* tile shape                           */
+    int threads;
+    ps_function standard;
+} ps_standard_plan;
+
+/* Run stages gts = gts_begin .. gts_end-1; stage gts reads absolute level
+ * its-2+gts and writes its-1+gts.  Replaces Solver.standardSolve and
+ * functions.StandardFunction / sendEdges (core/solver.py:178-189;
+ * core/functions.py:126-161). */
+int ps_standard_run(const ps_standard_plan *plan, int gts_begin, int gts_end);
+
+/* ---------------------------------------------------------------------- */
+/* Graph capture of the work stream (no counterpart in the reference).     */
+/* ---------------------------------------------------------------------- */
+int ps_graph_begin(void);
+int ps_graph_end(ps_graph *graph);
+int ps_graph_launch(ps_graph graph);
+int ps_graph_destroy(ps_graph graph);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYSWEEP_HIP_H */

@@ -1,0 +1,394 @@
+// libpysweep_hip.so -- HIP runtime, hiprtc and native phase sequencing for
+// pysweep_amd on MI355X (gfx950).  The C ABI is declared in
+// include/pysweep_hip.h; every entry returns 0 or an error code and keeps the
+// message for ps_last_error().
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pysweep_hip.h"
+
+namespace {
+
+thread_local std::string g_error;
+hipStream_t g_stream = nullptr;
+int g_device = -1;
+
+int fail(const std::string &what)
+{
+    g_error = what;
+    return 1;
+}
+
+int fail_hip(hipError_t e, const char *call, int line)
+{
+    char buf[512];
+    std::snprintf(buf, sizeof(buf), "%s failed at pysweep_hip.cpp:%d: %s (%d)", call, line,
+                  hipGetErrorString(e), static_cast<int>(e));
+    g_error = buf;
+    return static_cast<int>(e) ? static_cast<int>(e) : 1;
+}
+
+#define PS_HIP(call)                                              \
+    do {                                                          \
+        hipError_t e_ = (call);                                   \
+        if (e_ != hipSuccess) return fail_hip(e_, #call, __LINE__); \
+    } while (0)
+
+#define PS_NEED_INIT()                                                        \
+    do {                                                                      \
+        if (g_device < 0) return fail("ps_init() has not been called");       \
+    } while (0)
+
+char *dup_host(const std::string &s)
+{
+    char *p = static_cast<char *>(std::malloc(s.size() + 1));
+    if (p) std::memcpy(p, s.c_str(), s.size() + 1);
+    return p;
+}
+
+int launch(ps_function f, unsigned gx, unsigned threads, void **args)
+{
+    PS_HIP(hipModuleLaunchKernel(static_cast<hipFunction_t>(f), gx, 1, 1, threads, 1, 1, 0, g_stream,
+                                 args, nullptr));
+    return 0;
+}
+
+// One phase launch: K(state, gts, A0, grid).
+int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xoff)
+{
+    if (!f) return fail("swept plan has a null phase kernel");
+    ps_grid g;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.T = p->T;
+    g.xoff = xoff;
+    g.tiles_x = p->nx / p->b;
+    g.tiles_y = p->ny / p->b;
+    void *state = p->state;
+    void *args[] = {&state, &gts, &A0, &g};
+    return launch(f, static_cast<unsigned>(g.tiles_x * g.tiles_y), static_cast<unsigned>(p->threads), args);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ps_version(void) { return 1; }
+
+const char *ps_last_error(void) { return g_error.c_str(); }
+
+int ps_device_count(int *count)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+    return 0;
+}
+
+int ps_init(int device)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail("no HIP device is visible");
+    if (device < 0 || device >= n) return fail("device ordinal out of range");
+    if (g_device == device && g_stream) return 0;
+    PS_HIP(hipSetDevice(device));
+    if (g_stream) {
+        PS_HIP(hipStreamDestroy(g_stream));
+        g_stream = nullptr;
+    }
+    PS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
+    g_device = device;
+    return 0;
+}
+
+int ps_shutdown(void)
+{
+    if (g_stream) {
+        PS_HIP(hipStreamSynchronize(g_stream));
+        PS_HIP(hipStreamDestroy(g_stream));
+    }
+    g_stream = nullptr;
+    g_device = -1;
+    return 0;
+}
+
+int ps_compile(const char *source, const char *name, const char *const *options, int noptions, char **code,
+               size_t *size, char **log)
+{
+    *code = nullptr;
+    *size = 0;
+    if (log) *log = nullptr;
+    hiprtcProgram prog;
+    hiprtcResult r = hiprtcCreateProgram(&prog, source, name ? name : "pysweep.hip", 0, nullptr, nullptr);
+    if (r != HIPRTC_SUCCESS) return fail(std::string("hiprtcCreateProgram: ") + hiprtcGetErrorString(r));
+    r = hiprtcCompileProgram(prog, noptions, const_cast<const char **>(options));
+    size_t logsz = 0;
+    std::string logtxt;
+    if (hiprtcGetProgramLogSize(prog, &logsz) == HIPRTC_SUCCESS && logsz > 1) {
+        logtxt.resize(logsz);
+        hiprtcGetProgramLog(prog, &logtxt[0]);
+        while (!logtxt.empty() && logtxt.back() == '\0') logtxt.pop_back();
+    }
+    if (log && !logtxt.empty()) *log = dup_host(logtxt);
+    if (r != HIPRTC_SUCCESS) {
+        hiprtcDestroyProgram(&prog);
+        return fail(std::string("hiprtc compile of ") + (name ? name : "source") + " failed: " +
+                    hiprtcGetErrorString(r) + "\n" + logtxt);
+    }
+    size_t n = 0;
+    if (hiprtcGetCodeSize(prog, &n) != HIPRTC_SUCCESS || n == 0) {
+        hiprtcDestroyProgram(&prog);
+        return fail("hiprtc produced no code object");
+    }
+    char *buf = static_cast<char *>(std::malloc(n));
+    if (!buf) {
+        hiprtcDestroyProgram(&prog);
+        return fail("out of host memory for the code object");
+    }
+    hiprtcGetCode(prog, buf);
+    hiprtcDestroyProgram(&prog);
+    *code = buf;
+    *size = n;
+    return 0;
+}
+
+void ps_free_host(void *p) { std::free(p); }
+
+int ps_module_load(const void *code, size_t size, ps_module *module)
+{
+    PS_NEED_INIT();
+    (void)size;
+    hipModule_t m = nullptr;
+    PS_HIP(hipModuleLoadData(&m, code));
+    *module = m;
+    return 0;
+}
+
+int ps_module_unload(ps_module module)
+{
+    if (module) PS_HIP(hipModuleUnload(static_cast<hipModule_t>(module)));
+    return 0;
+}
+
+int ps_module_get_global(ps_module module, const char *name, void **dptr, size_t *bytes)
+{
+    hipDeviceptr_t p = nullptr;
+    size_t n = 0;
+    hipError_t e = hipModuleGetGlobal(&p, &n, static_cast<hipModule_t>(module), name);
+    if (e != hipSuccess) return fail(std::string("no device global named '") + name + "' in the module");
+    *dptr = p;
+    *bytes = n;
+    return 0;
+}
+
+int ps_module_get_function(ps_module module, const char *name, ps_function *function)
+{
+    hipFunction_t f = nullptr;
+    hipError_t e = hipModuleGetFunction(&f, static_cast<hipModule_t>(module), name);
+    if (e != hipSuccess) return fail(std::string("no kernel named '") + name + "' in the module");
+    *function = f;
+    return 0;
+}
+
+int ps_function_info(ps_function function, int *regs, int *lds_bytes, int *max_threads)
+{
+    hipFunction_t f = static_cast<hipFunction_t>(function);
+    int v = 0;
+    PS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_NUM_REGS, f));
+    *regs = v;
+    PS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_SHARED_SIZE_BYTES, f));
+    *lds_bytes = v;
+    PS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, f));
+    *max_threads = v;
+    return 0;
+}
+
+int ps_malloc(void **dptr, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMalloc(dptr, bytes));
+    return 0;
+}
+
+int ps_free(void *dptr)
+{
+    if (dptr) PS_HIP(hipFree(dptr));
+    return 0;
+}
+
+int ps_memcpy_htod(void *dst, const void *src, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    PS_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int ps_memcpy_dtoh(void *dst, const void *src, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    PS_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int ps_memcpy_dtod(void *dst, const void *src, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    PS_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int ps_memset(void *dst, int value, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemsetAsync(dst, value, bytes, g_stream));
+    return 0;
+}
+
+int ps_launch(ps_function function, unsigned gx, unsigned gy, unsigned gz, unsigned bx, unsigned by, unsigned bz,
+              unsigned shared_bytes, void **args)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipModuleLaunchKernel(static_cast<hipFunction_t>(function), gx, gy, gz, bx, by, bz, shared_bytes,
+                                 g_stream, args, nullptr));
+    return 0;
+}
+
+int ps_synchronize(void)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipStreamSynchronize(g_stream));
+    return 0;
+}
+
+int ps_event_create(ps_event *event)
+{
+    hipEvent_t e = nullptr;
+    PS_HIP(hipEventCreate(&e));
+    *event = e;
+    return 0;
+}
+
+int ps_event_record(ps_event event)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipEventRecord(static_cast<hipEvent_t>(event), g_stream));
+    return 0;
+}
+
+int ps_event_elapsed_ms(ps_event start, ps_event stop, float *ms)
+{
+    PS_HIP(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+    PS_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return 0;
+}
+
+int ps_event_destroy(ps_event event)
+{
+    if (event) PS_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return 0;
+}
+
+int ps_swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
+{
+    PS_NEED_INIT();
+    if (!p || !p->state) return fail("swept plan without state");
+    const int b = p->b, k = p->ops, s = p->its;
+    if (b <= 0 || k <= 0 || s <= 0 || b % (2 * k) != 0) return fail("invalid block size / operating points");
+    const int P = b / (2 * k) - 1;
+    if (P < 1) return fail("block size too small: b/(2k)-1 must be >= 1");
+    if (p->nx % b != 0 || p->ny % b != 0) return fail("nx and ny must be multiples of the block size");
+    if (p->T != 2 * P + s) return fail("ring must hold 2P+s levels");
+    if (G < 0 || c_begin < 0 || c_end > G + 2 || c_begin > c_end) return fail("cycle range out of bounds");
+    const int h = b / 2, base = s - 1;
+    for (int c = c_begin; c < c_end; ++c) {
+        const int xoff = (c % 2) * h;
+        int rc = 0;
+        if (c == 0) {
+            // FirstPrism: stages 1..P
+            if ((rc = launch_phase(p, p->up, 1, base, 0))) return rc;
+            if ((rc = launch_phase(p, p->ybridge, 1, base, 0))) return rc;
+        } else if (c <= G) {
+            // UpPrism c: stages (c-1)P+1 .. (c+1)P
+            const int gts = (c - 1) * P + 1;
+            const int A = base + (c - 1) * P;
+            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff))) return rc;
+            if ((rc = launch_phase(p, p->octahedron, gts, A, xoff))) return rc;
+            if ((rc = launch_phase(p, p->ybridge, gts + P, A + P, xoff))) return rc;
+        } else {
+            // LastPrism: stages GP+1 .. (G+1)P
+            const int gts = G * P + 1;
+            const int A = base + G * P;
+            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff))) return rc;
+            if ((rc = launch_phase(p, p->down, gts, A, xoff))) return rc;
+        }
+    }
+    return 0;
+}
+
+int ps_standard_run(const ps_standard_plan *p, int gts_begin, int gts_end)
+{
+    PS_NEED_INIT();
+    if (!p || !p->state || !p->standard) return fail("incomplete standard plan");
+    if (p->T < p->its + 1) return fail("standard ring must hold at least its+1 levels");
+    if (p->bx <= 0 || p->by <= 0 || p->bx > p->nx || p->by > p->ny) return fail("invalid standard tile");
+    ps_grid g;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.T = p->T;
+    g.xoff = 0;
+    g.tiles_x = (p->nx + p->bx - 1) / p->bx;
+    g.tiles_y = (p->ny + p->by - 1) / p->by;
+    void *state = p->state;
+    for (int gts = gts_begin; gts < gts_end; ++gts) {
+        int A = p->its - 2 + gts;
+        void *args[] = {&state, &gts, &A, &g};
+        int rc = launch(p->standard, static_cast<unsigned>(g.tiles_x * g.tiles_y),
+                        static_cast<unsigned>(p->threads), args);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int ps_graph_begin(void)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    return 0;
+}
+
+int ps_graph_end(ps_graph *graph)
+{
+    PS_NEED_INIT();
+    hipGraph_t g = nullptr;
+    PS_HIP(hipStreamEndCapture(g_stream, &g));
+    hipGraphExec_t ge = nullptr;
+    hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return fail_hip(e, "hipGraphInstantiate", __LINE__);
+    *graph = ge;
+    return 0;
+}
+
+int ps_graph_launch(ps_graph graph)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), g_stream));
+    return 0;
+}
+
+int ps_graph_destroy(ps_graph graph)
+{
+    if (graph) PS_HIP(hipGraphExecDestroy(static_cast<hipGraphExec_t>(graph)));
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+"""Device-resident swept and standard solves for one plugin on one GPU.
+
+The state lives in HBM as a ring ``state[T][nv][nx][ny]`` (float64, y
+contiguous); absolute level A is slot A % T.  Nothing is copied between
+phases: the reference's host gather / H2D / D2H / scatter per phase group
+(core/functions.py:21-33) and its window roll and shift (functions.py:90-123,
+io.py:134-144) are replaced by ring indexing and a per-cycle lattice offset
+inside the kernels.
+"""
+import ctypes
+
+import numpy
+
+from pysweep_amd import hip, kernels, plugins, schedule
+
+
+class _Run:
+    def __init__(self, nv, its, nx, ny, T):
+        self.nv, self.its, self.nx, self.ny, self.T = nv, its, nx, ny, T
+        self.level_bytes = nv * nx * ny * 8
+        self.ring = hip.DeviceArray(T * self.level_bytes)
+        self.modules = []
+
+    def facade(self):
+        return plugins.SourceModuleFacade(self.modules)
+
+    def set_globals(self, cpu_module, globals_):
+        """The plugin writes its device constants into every module."""
+        cpu_module.set_globals(*globals_, source_mod=self.facade())
+
+    def upload_ic(self, ic):
+        ic = numpy.ascontiguousarray(ic, dtype=numpy.float64)
+        if ic.shape != (self.nv, self.nx, self.ny):
+            raise ValueError(f"initial condition shape {ic.shape} != {(self.nv, self.nx, self.ny)}")
+        for A in range(self.its):
+            self.ring.upload(ic, (A % self.T) * self.level_bytes)
+
+    def level(self, A, out=None):
+        out = numpy.empty((self.nv, self.nx, self.ny)) if out is None else out
+        return self.ring.download(out, (A % self.T) * self.level_bytes)
+
+    def close(self):
+        self.ring.close()
+
+
+class StandardRun(_Run):
+    """Standard decomposition: one launch of ``Standard`` per stage."""
+
+    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, T=None):
+        super().__init__(nv, its, nx, ny, T or its + 1)
+        self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads)
+        mod = kernels.load(self.spec)
+        self.modules = [mod]
+        self.plan = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=self.spec.bx,
+                                     by=self.spec.by, threads=self.spec.threads,
+                                     standard=mod.get_function("Standard"))
+
+    def run(self, g_begin, g_end, on_level=None):
+        """Stages g_begin..g_end-1; ``on_level(A)`` after each full step."""
+        if on_level is None:
+            hip.standard_run(self.plan, g_begin, g_end)
+            return
+        for g in range(g_begin, g_end):
+            hip.standard_run(self.plan, g, g + 1)
+            A = self.its - 1 + g
+            if (A - (self.its - 1)) % self.its == 0:
+                on_level(A)
+
+
+class SweptRun(_Run):
+    """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
+
+    def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True):
+        P = b // (2 * ops) - 1
+        if b % (2 * ops) or P < 1:
+            raise ValueError(f"invalid blocksize {b} for operating_points {ops}")
+        if nx % b or ny % b:
+            raise ValueError(f"grid {nx}x{ny} is not divisible by the blocksize {b}")
+        super().__init__(nv, its, nx, ny, 2 * P + its)
+        self.P, self.b, self.ops = P, b, ops
+        self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads)
+        mod = kernels.load(self.spec)
+        self.modules = [mod]
+        f = {n: mod.get_function(n) for n in kernels.SWEPT_KERNELS}
+        self.plan = hip.SweptPlan(state=self.ring.ptr, nx=nx, ny=ny, b=b, ops=ops, its=its, T=self.T,
+                                  threads=self.spec.threads, up=f["UpPyramid"], ybridge=f["YBridge"],
+                                  xbridge=f["XBridge"], octahedron=f["Octahedron"], down=f["DownPyramid"])
+        self.tail = None
+        if tail_standard:
+            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny)
+            smod = kernels.load(sspec)
+            self.modules.append(smod)
+            self.tail = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=sspec.bx,
+                                         by=sspec.by, threads=sspec.threads, standard=smod.get_function("Standard"))
+
+    def run(self, times, on_level=None):
+        """Run the swept cycles of ``times`` (schedule.SweptTimes) and then
+        its tail stages; ``on_level(A)`` for every full-step level as soon as
+        it is complete.  Returns the final absolute level."""
+        G, P, s = times.G, self.P, self.its
+        done = s - 1
+        if G >= 0:
+            if on_level is None:
+                hip.swept_run(self.plan, G, 0, G + 2)
+                done = schedule.complete_after_cycle(G + 1, G, P, s)
+            else:
+                for c in range(G + 2):
+                    hip.swept_run(self.plan, G, c, c + 1)
+                    hi = schedule.complete_after_cycle(c, G, P, s)
+                    for A in schedule.full_step_levels(s, done, hi):
+                        on_level(A)
+                    done = hi
+        if times.tail:
+            if self.tail is None:
+                raise RuntimeError("exact-step tail needs the standard kernel")
+            g0 = done - (s - 1) + 1
+            for g in range(g0, g0 + times.tail):
+                hip.standard_run(self.tail, g, g + 1)
+                A = s - 1 + g
+                if on_level is not None and (A - (s - 1)) % s == 0:
+                    on_level(A)
+            done += times.tail
+        return done
+
+
+def launch_info(run):
+    """Registers / LDS of the loaded kernels (for reports)."""
+    out = {}
+    for m in run.modules:
+        for name in ("UpPyramid", "Octahedron", "Standard"):
+            try:
+                out[name] = hip.function_info(m.get_function(name))
+            except Exception:
+                pass
+    return out

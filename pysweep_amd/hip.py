@@ -1,0 +1,305 @@
+"""ctypes binding of libpysweep_hip.so (C ABI in include/pysweep_hip.h).
+
+This is the only module that talks to the native library.  It raises
+``ImportError`` at import time when the library has not been built, and
+``HipError`` whenever an entry point reports a failure; there is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+import numpy
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBDIR = os.path.join(HERE, "_lib")
+LIBPATH = os.path.join(LIBDIR, "libpysweep_hip.so")
+
+
+class HipError(RuntimeError):
+    pass
+
+
+if not os.path.exists(LIBPATH):
+    raise ImportError(f"{LIBPATH} is missing: build it with __graft_entry__.build() "
+                      "or python -m pysweep_amd.build")
+
+_lib = ctypes.CDLL(LIBPATH)
+
+c_int, c_size_t, c_void_p, c_char_p = ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_char_p
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("nx", c_int), ("ny", c_int), ("T", c_int), ("xoff", c_int),
+                ("tiles_x", c_int), ("tiles_y", c_int)]
+
+
+class SweptPlan(ctypes.Structure):
+    _fields_ = [("state", c_void_p), ("nx", c_int), ("ny", c_int), ("b", c_int), ("ops", c_int),
+                ("its", c_int), ("T", c_int), ("threads", c_int),
+                ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p),
+                ("octahedron", c_void_p), ("down", c_void_p)]
+
+
+class StandardPlan(ctypes.Structure):
+    _fields_ = [("state", c_void_p), ("nx", c_int), ("ny", c_int), ("its", c_int), ("T", c_int),
+                ("bx", c_int), ("by", c_int), ("threads", c_int), ("standard", c_void_p)]
+
+
+def _sig(name, *argtypes, restype=c_int):
+    f = getattr(_lib, name)
+    f.argtypes = list(argtypes)
+    f.restype = restype
+    return f
+
+
+_lib.ps_last_error.restype = c_char_p
+_lib.ps_last_error.argtypes = []
+_version = _sig("ps_version")
+_device_count = _sig("ps_device_count", ctypes.POINTER(c_int))
+_init = _sig("ps_init", c_int)
+_shutdown = _sig("ps_shutdown")
+_compile = _sig("ps_compile", c_char_p, c_char_p, ctypes.POINTER(c_char_p), c_int,
+                ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p))
+_free_host = _sig("ps_free_host", c_void_p, restype=None)
+_module_load = _sig("ps_module_load", c_void_p, c_size_t, ctypes.POINTER(c_void_p))
+_module_unload = _sig("ps_module_unload", c_void_p)
+_get_global = _sig("ps_module_get_global", c_void_p, c_char_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t))
+_get_function = _sig("ps_module_get_function", c_void_p, c_char_p, ctypes.POINTER(c_void_p))
+_function_info = _sig("ps_function_info", c_void_p, ctypes.POINTER(c_int), ctypes.POINTER(c_int),
+                      ctypes.POINTER(c_int))
+_malloc = _sig("ps_malloc", ctypes.POINTER(c_void_p), c_size_t)
+_free = _sig("ps_free", c_void_p)
+_htod = _sig("ps_memcpy_htod", c_void_p, c_void_p, c_size_t)
+_dtoh = _sig("ps_memcpy_dtoh", c_void_p, c_void_p, c_size_t)
+_dtod = _sig("ps_memcpy_dtod", c_void_p, c_void_p, c_size_t)
+_memset = _sig("ps_memset", c_void_p, c_int, c_size_t)
+_launch = _sig("ps_launch", c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
+               ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(c_void_p))
+_sync = _sig("ps_synchronize")
+_event_create = _sig("ps_event_create", ctypes.POINTER(c_void_p))
+_event_record = _sig("ps_event_record", c_void_p)
+_event_elapsed = _sig("ps_event_elapsed_ms", c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float))
+_event_destroy = _sig("ps_event_destroy", c_void_p)
+_swept_run = _sig("ps_swept_run", ctypes.POINTER(SweptPlan), c_int, c_int, c_int)
+_standard_run = _sig("ps_standard_run", ctypes.POINTER(StandardPlan), c_int, c_int)
+_graph_begin = _sig("ps_graph_begin")
+_graph_end = _sig("ps_graph_end", ctypes.POINTER(c_void_p))
+_graph_launch = _sig("ps_graph_launch", c_void_p)
+_graph_destroy = _sig("ps_graph_destroy", c_void_p)
+
+#: every symbol the header declares (checked by the CPU test suite)
+EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shutdown", "ps_compile",
+           "ps_free_host", "ps_module_load", "ps_module_unload", "ps_module_get_global",
+           "ps_module_get_function", "ps_function_info", "ps_malloc", "ps_free", "ps_memcpy_htod",
+           "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_launch", "ps_synchronize",
+           "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_destroy",
+           "ps_swept_run", "ps_standard_run", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
+           "ps_graph_destroy")
+
+
+def last_error():
+    return _lib.ps_last_error().decode(errors="replace")
+
+
+def check(rc, what=""):
+    if rc != 0:
+        raise HipError(f"{what}: {last_error()}" if what else last_error())
+
+
+def version():
+    return _version()
+
+
+def device_count():
+    n = c_int(0)
+    check(_device_count(ctypes.byref(n)), "ps_device_count")
+    return n.value
+
+
+_initialised = False
+
+
+def init(device=0):
+    global _initialised
+    check(_init(device), "ps_init")
+    _initialised = True
+
+
+def ensure_init(device=0):
+    if not _initialised:
+        init(device)
+
+
+def shutdown():
+    global _initialised
+    check(_shutdown(), "ps_shutdown")
+    _initialised = False
+
+
+def compile_source(source, name, options):
+    """hiprtc-compile source text; returns (code bytes, log text)."""
+    opts = (c_char_p * len(options))(*[o.encode() for o in options])
+    code, size, log = c_void_p(), c_size_t(), c_void_p()
+    rc = _compile(source.encode(), name.encode(), opts, len(options), ctypes.byref(code),
+                  ctypes.byref(size), ctypes.byref(log))
+    logtxt = ""
+    if log.value:
+        logtxt = ctypes.string_at(log.value).decode(errors="replace")
+        _free_host(log)
+    if rc != 0:
+        raise HipError(last_error())
+    data = ctypes.string_at(code.value, size.value)
+    _free_host(code)
+    return data, logtxt
+
+
+class Module:
+    """A loaded code object."""
+
+    def __init__(self, code):
+        ensure_init()
+        self._code = code  # kept alive for the runtime
+        h = c_void_p()
+        check(_module_load(code, len(code), ctypes.byref(h)), "ps_module_load")
+        self.handle = h.value
+
+    def get_function(self, name):
+        f = c_void_p()
+        check(_get_function(self.handle, name.encode(), ctypes.byref(f)), "ps_module_get_function")
+        return f.value
+
+    def get_global(self, name):
+        p, n = c_void_p(), c_size_t()
+        check(_get_global(self.handle, name.encode(), ctypes.byref(p), ctypes.byref(n)), "ps_module_get_global")
+        return p.value, n.value
+
+    def unload(self):
+        if self.handle:
+            _module_unload(self.handle)
+            self.handle = None
+
+
+def function_info(fn):
+    r, l, m = c_int(), c_int(), c_int()
+    check(_function_info(fn, ctypes.byref(r), ctypes.byref(l), ctypes.byref(m)), "ps_function_info")
+    return {"regs": r.value, "lds_bytes": l.value, "max_threads": m.value}
+
+
+class DeviceArray:
+    """Owned device allocation (freed on close / garbage collection)."""
+
+    def __init__(self, nbytes):
+        ensure_init()
+        p = c_void_p()
+        check(_malloc(ctypes.byref(p), max(int(nbytes), 1)), "ps_malloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def upload(self, arr, offset=0):
+        a = numpy.ascontiguousarray(arr)
+        assert offset + a.nbytes <= self.nbytes
+        check(_htod(self.ptr + offset, a.ctypes.data, a.nbytes), "ps_memcpy_htod")
+
+    def download(self, out, offset=0):
+        assert out.flags.c_contiguous and offset + out.nbytes <= self.nbytes
+        check(_dtoh(out.ctypes.data, self.ptr + offset, out.nbytes), "ps_memcpy_dtoh")
+        return out
+
+    def fill_zero(self):
+        check(_memset(self.ptr, 0, self.nbytes), "ps_memset")
+
+    def close(self):
+        if self.ptr:
+            _free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def htod(dptr, arr):
+    a = numpy.ascontiguousarray(arr)
+    check(_htod(dptr, a.ctypes.data, a.nbytes), "ps_memcpy_htod")
+
+
+def htod_bytes(dptr, data):
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    check(_htod(dptr, ctypes.addressof(buf), len(data)), "ps_memcpy_htod")
+
+
+def dtod(dst, src, nbytes):
+    check(_dtod(dst, src, nbytes), "ps_memcpy_dtod")
+
+
+def synchronize():
+    check(_sync(), "ps_synchronize")
+
+
+def launch(fn, grid, block, args):
+    """args: list of ctypes objects (kernel arguments in order)."""
+    arr = (c_void_p * len(args))(*[ctypes.cast(ctypes.pointer(a), c_void_p) for a in args])
+    g = tuple(grid) + (1,) * (3 - len(grid))
+    b = tuple(block) + (1,) * (3 - len(block))
+    check(_launch(fn, g[0], g[1], g[2], b[0], b[1], b[2], 0, arr), "ps_launch")
+
+
+def swept_run(plan, G, c_begin, c_end):
+    check(_swept_run(ctypes.byref(plan), G, c_begin, c_end), "ps_swept_run")
+
+
+def standard_run(plan, g_begin, g_end):
+    check(_standard_run(ctypes.byref(plan), g_begin, g_end), "ps_standard_run")
+
+
+class Event:
+    def __init__(self):
+        h = c_void_p()
+        check(_event_create(ctypes.byref(h)), "ps_event_create")
+        self.h = h.value
+
+    def record(self):
+        check(_event_record(self.h), "ps_event_record")
+
+    def elapsed_ms(self, stop):
+        ms = ctypes.c_float()
+        check(_event_elapsed(self.h, stop.h, ctypes.byref(ms)), "ps_event_elapsed_ms")
+        return ms.value
+
+    def __del__(self):
+        try:
+            _event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class Graph:
+    """Capture of the work stream: ``with Graph() as g: <launches>``; then
+    ``g.launch()`` replays them."""
+
+    def __init__(self):
+        self.h = None
+
+    def __enter__(self):
+        check(_graph_begin(), "ps_graph_begin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        h = c_void_p()
+        rc = _graph_end(ctypes.byref(h))
+        if exc_type is None:
+            check(rc, "ps_graph_end")
+            self.h = h.value
+        return False
+
+    def launch(self):
+        check(_graph_launch(self.h), "ps_graph_launch")
+
+    def __del__(self):
+        try:
+            if self.h:
+                _graph_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,145 @@
+"""Kernel module specifications, launch-shape defaults and loading.
+
+A ``ModuleSpec`` fixes everything compiled into one hiprtc module: the
+plugin's device source, the stencil geometry (NV, OPS, ITS), and the tile
+shape / LDS window / workgroup size.  ``build_module_code`` compiles (or
+fetches from the cache) without a GPU; ``load`` needs one.
+"""
+import dataclasses
+import os
+
+from pysweep_amd import compiler
+from pysweep_amd.equations import path as eq_path
+
+#: LDS per workgroup the default shapes aim for; 160 KB per CU leaves room
+#: for two workgroups (eight wavefronts) per CU.
+LDS_BUDGET = 64 * 1024
+LDS_MAX = 160 * 1024
+
+
+@dataclasses.dataclass(frozen=True)
+class ModuleSpec:
+    kind: str            # "swept" or "standard"
+    user_path: str       # plugin device source
+    nv: int
+    ops: int
+    its: int
+    b: int = 0           # swept tile edge
+    slots: int = 0       # swept LDS level slots
+    bx: int = 0          # standard tile rows
+    by: int = 0          # standard tile columns
+    threads: int = 256
+
+    def source(self):
+        with open(self.user_path) as f:
+            user = f.read()
+        name = os.path.basename(self.user_path)
+        if self.kind == "swept":
+            return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads)
+        if self.kind == "standard":
+            return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads)
+        raise ValueError(self.kind)
+
+    def lds_bytes(self):
+        if self.kind == "swept":
+            return self.slots * self.nv * self.b * self.b * 8
+        return (self.its + 1) * self.nv * (self.bx + 2 * self.ops) * (self.by + 2 * self.ops) * 8
+
+    def describe(self):
+        g = (f"b={self.b} slots={self.slots}" if self.kind == "swept" else f"tile={self.bx}x{self.by}")
+        return f"{self.kind} {os.path.basename(self.user_path)} nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}"
+
+
+def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET):
+    if b % (2 * ops) != 0:
+        raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
+    P = b // (2 * ops) - 1
+    if P < 1:
+        raise ValueError(f"blocksize {b} too small for operating_points={ops} (needs b >= {4 * ops})")
+    level = nv * b * b * 8
+    full = 2 * P + its
+    if slots is None:
+        slots = min(full, max(its + 1, lds_budget // level))
+    if slots < its + 1:
+        raise ValueError("an LDS window needs at least intermediate_steps+1 levels")
+    if slots * level > LDS_MAX:
+        raise ValueError(f"swept tile b={b} with {slots} levels needs {slots * level} B of LDS (> {LDS_MAX})")
+    if threads is None:
+        threads = max(64, min(256, (b * b + 63) // 64 * 64))
+    return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads)
+
+
+def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET):
+    """Tile (bx rows, by columns) for the standard kernel: most cells within
+    the LDS budget, then rows of at least 32 doubles (coalesced wavefront
+    loads), then the least halo re-reading."""
+    best = None
+    for by in (64, 32, 16, 8):
+        for bx in (32, 16, 8, 4, 2):
+            if bx > nx or by > ny:
+                continue
+            lds = (its + 1) * nv * (bx + 2 * ops) * (by + 2 * ops) * 8
+            if lds > lds_budget:
+                continue
+            halo = (bx + 2 * ops) * (by + 2 * ops) / (bx * by)
+            score = (bx * by, by >= 32, -halo)
+            if best is None or score > best[0]:
+                best = (score, bx, by)
+    if best is None:
+        return max(1, min(nx, 4)), max(1, min(ny, 16))
+    return best[1], best[2]
+
+
+def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, lds_budget=LDS_BUDGET):
+    if bx is None or by is None:
+        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget)
+    if bx + 2 * ops > nx + 2 * ops and bx > nx:
+        raise ValueError("standard tile larger than the grid")
+    return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads)
+
+
+def build_module_code(spec):
+    return compiler.compile_cached(spec.source(), os.path.basename(spec.user_path) + f".{spec.kind}")
+
+
+_loaded = {}
+
+
+def load(spec):
+    """Load (once per process) the module for ``spec``; returns hip.Module."""
+    from pysweep_amd import hip
+    m = _loaded.get(spec)
+    if m is None:
+        m = hip.Module(build_module_code(spec))
+        _loaded[spec] = m
+    return m
+
+
+SWEPT_KERNELS = ("UpPyramid", "YBridge", "XBridge", "Octahedron", "DownPyramid")
+
+
+def shipped_module_specs():
+    """Modules used by the test suite, smoke() and bench.py."""
+    specs = []
+    heat, euler = eq_path("heat.cu"), eq_path("euler.cu")
+    ex, ch, hf = eq_path("example.cu"), eq_path("checker.cu"), eq_path("half.cu")
+    for b in (8, 16, 32):
+        specs.append(swept_spec(heat, 1, 1, 1, b))
+        specs.append(swept_spec(heat, 1, 1, 2, b))
+    for b in (8, 16, 32):
+        specs.append(swept_spec(euler, 4, 2, 2, b))
+    specs.append(swept_spec(ex, 1, 1, 1, 8))
+    specs.append(swept_spec(ex, 1, 2, 2, 16))
+    specs.append(swept_spec(ch, 1, 1, 1, 8))
+    specs.append(swept_spec(ch, 1, 2, 2, 16))
+    specs.append(swept_spec(hf, 1, 1, 2, 8))
+    for (path, nv, k, s) in ((heat, 1, 1, 1), (heat, 1, 1, 2), (euler, 4, 2, 2), (ex, 1, 1, 1), (ex, 1, 2, 2),
+                             (ch, 1, 1, 1), (ch, 1, 2, 2), (hf, 1, 1, 2)):
+        for n in (64, 384, 1024):
+            specs.append(standard_spec(path, nv, k, s, n, n))
+    seen, out = set(), []
+    for s in specs:
+        if s not in seen:
+            seen.add(s)
+            out.append(s)
+    return out

@@ -1,0 +1,76 @@
+"""Time-step bookkeeping of the swept and standard solvers.
+
+Restates createTimeStepData (core/solver.py:113-128) and the output-level
+selection of io.sweptWrite / io.standardWrite (core/io.py:134-144, 186-208)
+for a device-resident ring of levels:
+
+* absolute level A holds the state after A-(s-1) stages; levels 0..s-1 are
+  the initial condition (core/block.py:25-26); stage g (globalTimeStep g,
+  1-based) reads level s-2+g and writes s-1+g;
+* full time steps are the levels A = s-1 + m*s, m = 0, 1, ...
+"""
+import dataclasses
+
+
+@dataclasses.dataclass
+class SweptTimes:
+    N: int            # requested steps, int((tf-t0)/dt)
+    P: int            # max pyramid swept steps, b/(2k)-1
+    G: int            # UpPrism cycles
+    stages: int       # swept stages, P*(G+1)
+    tail: int         # standard stages run after the swept cycles (exact mode)
+    nt: int           # full-step levels recorded
+    subtraction: int  # the reference's output-parity flag
+    window: int       # ring levels, 2P+s
+
+
+def requested_steps(t0, tf, dt):
+    """N = int((tf - t0)/dt) (core/solver.py:115)."""
+    return int((tf - t0) / dt)
+
+
+def swept_times(N, b, k, s, exact=False):
+    """Cycle count and stage totals.
+
+    Reference mode (exact=False): G = int(s*N/P - 1), P*(G+1) stages and
+    nt = int(P*(G+1)/s + 1) recorded levels, which quantizes the run to a
+    multiple of P stages (core/solver.py:119-121).
+
+    Exact mode: the largest G with P*(G+1) <= s*N, then s*N - P*(G+1) stages
+    of the standard kernel on the same ring, so exactly N steps are taken
+    (G = -1 means no swept cycle fits and every stage is standard).
+    """
+    P = b // (2 * k) - 1
+    if P < 1:
+        raise ValueError(f"blocksize {b} gives no pyramid for operating_points={k}")
+    if exact:
+        G = (s * N) // P - 1
+        swept = P * (G + 1) if G >= 0 else 0
+        tail = s * N - swept
+        nt = N + 1
+    else:
+        G = int(s * N / P - 1)
+        if G < 0:
+            raise ValueError("fewer stages than one pyramid; use exact mode or a smaller blocksize")
+        swept = P * (G + 1)
+        tail = 0
+        nt = int(P * (G + 1) / s + 1)
+    sub = 1 if (P % 2 != 0 and s % 2 == 0) else 0
+    return SweptTimes(N=N, P=P, G=G, stages=swept, tail=tail, nt=nt, subtraction=sub, window=2 * P + s)
+
+
+def complete_after_cycle(c, G, P, s):
+    """Highest absolute level whose every cell is final after swept cycle c
+    (0 = FirstPrism, 1..G = UpPrism, G+1 = LastPrism)."""
+    if c == 0:
+        return s - 1
+    return s - 1 + min(c, G + 1) * P
+
+
+def full_step_levels(s, lo, hi):
+    """Full-step absolute levels A in (lo, hi]."""
+    first = lo + 1
+    r = (first - (s - 1)) % s
+    if r:
+        first += s - r
+    return list(range(first, hi + 1, s))

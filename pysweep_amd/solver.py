@@ -1,0 +1,246 @@
+"""``Solver``: the reference's user-facing object (core/solver.py:9-189) on
+one MI355X.
+
+Same construction and attribute API as the reference:
+
+    solver = Solver(initialConditions, "input.yaml")   # or set attributes
+    solver()                                           # run
+
+Attributes (core/io.py:35-75): simulation (True = swept), verbose,
+intermediate, operating, blocksize (b, b, 1), share, exid, globals
+[t0, tf, dt, dx, dy, *equation args], cpu / gpu plugin paths, output,
+dtypeStr / dtype.  The MPI split (core/process.py) collapses to one rank
+with the whole grid on the GPU, so ``share`` must be 1.
+
+Additions: ``exact`` (take exactly N steps instead of the reference's
+quantization to multiples of the pyramid height), ``history`` (keep every
+full-step level, the reference's output dataset ``data``), ``result`` (final
+full-step state), ``solve_seconds`` and ``gcups``.
+"""
+import os
+import time
+import warnings
+
+import numpy
+
+from pysweep_amd import config, plugins, schedule
+
+
+class Solver(object):
+    """Swept / standard solver of a plugin pair on one GPU."""
+
+    def __init__(self, initialConditions=None, yamlFileName=None, sendWarning=True):
+        super().__init__()
+        self.moments = [time.time()]
+        # single process: rank 0 is node master and cluster master
+        self.rank = 0
+        self.nodeMasterBool = self.clusterMasterBool = True
+        self.simulation = True
+        self.verbose = False
+        self.intermediate = 1
+        self.operating = 1
+        self.blocksize = (32, 32, 1)
+        self.share = 1.0
+        self.exid = []
+        self.globals = []
+        self.cpu = self.cpuStr = None
+        self.gpu = self.gpuStr = None
+        self.output = "output.hdf5"
+        self.dtypeStr = "float64"
+        self.dtype = numpy.dtype(self.dtypeStr)
+        self.exact = False
+        self.history = False
+        self.write_output = False
+        self.logfile = None
+        self.tuning = {}
+        self.result = None
+        self.data = None
+        self.solve_seconds = None
+        self.assignInitialConditions(initialConditions, sendWarning=sendWarning)
+        if yamlFileName is not None:
+            self.yamlFileName = yamlFileName
+            config.yamlManager(self)
+        elif sendWarning:
+            warnings.warn("yaml not specified, requires manual input.")
+
+    # ------------------------------------------------------------------ API
+    def __call__(self, start=0, stop=-1, libname=None, recompile=False):
+        """Run the solve (core/solver.py:25-70)."""
+        self.moments.append(time.time())
+        self._verbose("-----------------------------PySweep-------------------------\n")
+        self._check_inputs()
+        self.createTimeStepData()
+        self.moments.append(time.time())
+        self._setup_device()
+        self.moments.append(time.time())
+        self._verbose(str(self))
+        if self.simulation:
+            self.sweptSolve()
+        else:
+            self.standardSolve()
+        self.moments.append(time.time())
+        self._finish(self.moments[start], self.moments[stop])
+
+    def __str__(self):
+        from pysweep_amd import reporting
+        return reporting.solver_print(self)
+
+    def compactPrint(self):
+        """core/solver.py:76-81."""
+        short = "swept: " if self.simulation else "standard: "
+        moment = self.moments[-1] - self.moments[0] if self.moments else None
+        short += "{}, {}, {}, {}, {}, {}".format(self.blocksize[0], self.share, self.intermediate, self.operating,
+                                                  getattr(self, "timeSteps", None), moment)
+        print(short)
+
+    def loadCPUModule(self):
+        """Import the plugin's Python half (core/solver.py:83-87)."""
+        self.cpu = plugins.load_cpu_module(self.cpu)
+
+    def assignInitialConditions(self, initialConditions, sendWarning=True):
+        """Array (nv, nx, ny) or HDF5 file with dataset ``data`` (core/solver.py:89-101)."""
+        if isinstance(initialConditions, str):
+            from pysweep_amd import hdf5io
+            self.initialConditions = hdf5io.read_initial_conditions(initialConditions)
+            self.arrayShape = self.initialConditions.shape
+        elif initialConditions is not None:
+            self.initialConditions = numpy.asarray(initialConditions)
+            self.arrayShape = numpy.shape(initialConditions)
+        elif sendWarning:
+            warnings.warn("Initial condition not provided, requires manual assignment with "
+                          "assignInitialConditionsFunction.")
+
+    def setGPU(self, gpu):
+        self.gpu = gpu
+        self.gpuStr = gpu
+
+    def setCPU(self, cpu):
+        self.cpu = cpu
+        self.cpuStr = cpu
+
+    def createTimeStepData(self):
+        """core/solver.py:113-128, plus the exact-step option."""
+        t0, tf, dt = self.globals[0], self.globals[1], self.globals[2]
+        self.timeSteps = schedule.requested_steps(t0, tf, dt)
+        nv, nx, ny = self.initialConditions.shape
+        self.globalShape = (nv, nx, ny)
+        s = self.intermediate
+        if self.simulation:
+            b = self.blocksize[0]
+            self.splitx = self.splity = b // 2
+            self.times = schedule.swept_times(self.timeSteps, b, self.operating, s, exact=self.exact)
+            self.maxPyramidSize = self.times.P
+            self.maxGlobalSweptStep = self.times.G
+            self.maxOctSize = 2 * self.times.P
+            self.subtraction = self.times.subtraction
+            self.sharedShape = (self.times.window, nv, nx, ny)
+            self.arrayShape = (self.times.nt, nv, nx, ny)
+            self.stages = self.times.stages + self.times.tail
+        else:
+            self.sharedShape = (s + 1, nv, nx, ny)
+            self.arrayShape = (self.timeSteps + 1, nv, nx, ny)
+            self.stages = s * self.timeSteps
+
+    def sweptSolve(self):
+        """FirstPrism, G UpPrism cycles, LastPrism (core/solver.py:158-176)."""
+        from pysweep_amd import hip
+        run = self._run
+        collect = self._collector()
+        start, stop = hip.Event(), hip.Event()
+        start.record()
+        self.final_level = run.run(self.times, on_level=collect)
+        stop.record()
+        self.solve_seconds = start.elapsed_ms(stop) / 1e3
+        self.globalTimeStep = self.final_level - (self.intermediate - 1) + 1
+
+    def standardSolve(self):
+        """One stage per launch (core/solver.py:178-189)."""
+        from pysweep_amd import hip
+        run = self._run
+        collect = self._collector()
+        start, stop = hip.Event(), hip.Event()
+        start.record()
+        run.run(1, self.stages + 1, on_level=collect)
+        stop.record()
+        self.solve_seconds = start.elapsed_ms(stop) / 1e3
+        self.final_level = self.intermediate - 1 + self.stages
+        self.globalTimeStep = self.stages + 1
+
+    # ------------------------------------------------------------ internals
+    def _verbose(self, msg):
+        if self.verbose:
+            print(msg)
+
+    def _check_inputs(self):
+        if self.initialConditions is None:
+            raise ValueError("no initial conditions")
+        if len(self.initialConditions.shape) != 3:
+            raise ValueError("initial conditions must be an array (nv, nx, ny)")
+        if self.share != 1 and self.share != 1.0:
+            raise NotImplementedError(
+                "this build runs the whole grid on one GPU (share=1); the CPU/GPU split is not supported")
+        if self.dtype != numpy.float64:
+            raise NotImplementedError("device state is float64 (set dtype: float64)")
+        b = self.blocksize[0]
+        assert b % (2 * self.operating) == 0, ("Invalid blocksize, blocksize must satisfy BS = 2*ops*k and the "
+                                               "architectural limit where k is any integer factor.")
+        nv, nx, ny = self.initialConditions.shape
+        if self.simulation:
+            assert nx % b == 0 and ny % b == 0, \
+                "Provided array dimensions is not divisible by the specified block size."
+        if isinstance(self.cpu, str):
+            self.loadCPUModule()
+        if self.cpu is None or self.gpu is None:
+            raise ValueError("cpu and gpu plugin files must be set")
+
+    def _setup_device(self):
+        from pysweep_amd import device, hip
+        hip.ensure_init(0)
+        nv, nx, ny = self.initialConditions.shape
+        t = dict(self.tuning)
+        if self.simulation:
+            self._run = device.SweptRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
+                                        self.blocksize[0], slots=t.get("slots"), threads=t.get("threads"),
+                                        tail_standard=True)
+        else:
+            self._run = device.StandardRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
+                                           bx=t.get("bx"), by=t.get("by"), threads=t.get("threads", 256))
+        # CPU globals, then the plugin's device constants (core/block.py:50, 90)
+        self.cpu.set_globals(*self.globals)
+        self._run.set_globals(self.cpu, self.globals)
+        self._run.upload_ic(self.initialConditions)
+
+    def _collector(self):
+        s = self.intermediate
+        if not (self.history or self.write_output):
+            return None
+        nt = self.arrayShape[0]
+        self.data = numpy.empty(self.arrayShape)
+        self.data[0] = self.initialConditions
+        run = self._run
+
+        def on_level(A):
+            m = (A - (s - 1)) // s
+            if m < nt:
+                run.level(A, out=self.data[m])
+
+        return on_level
+
+    def _finish(self, t_start, t_stop):
+        s = self.intermediate
+        last_full = self.final_level - ((self.final_level - (s - 1)) % s)
+        self.result = self._run.level(last_full)
+        self.result_step = (last_full - (s - 1)) // s
+        self.result_level = self.final_level
+        self.final_state = self._run.level(self.final_level)
+        full_steps = self.stages / s
+        nv, nx, ny = self.globalShape
+        self.gcups = nx * ny * full_steps / self.solve_seconds / 1e9 if self.solve_seconds else None
+        self.runtime = t_stop - t_start
+        if self.write_output and self.data is not None:
+            from pysweep_amd import hdf5io
+            hdf5io.write_output(self, self.output)
+        if self.logfile:
+            from pysweep_amd import reporting
+            reporting.update_log_file(self, self.runtime, self.logfile)
+        self._verbose("Done in {} seconds...\n".format(self.moments[-1] - self.moments[0]))

@@ -1,0 +1,198 @@
+"""GPU parity: the HIP swept and standard kernels (through the C ABI and the
+product Solver) against the CPU oracle on the same seeded inputs.
+
+Tolerance: max|gpu - oracle| / max|oracle| <= 1e-12 per variable (FP64).
+The device plugins evaluate the numpy plugin's operation order with no FMA
+contraction, so results are also expected to be bitwise equal; for Euler
+that is what makes multi-stage agreement possible at all (the scheme turns
+a 1-ulp difference into >1e-10 within a few stages, see
+tests/test_oracle_euler.py::test_round_off_sensitivity_is_a_property_of_the_scheme).
+"""
+import numpy as np
+import pytest
+
+from gpu_helpers import (euler_case, gpu_solve, heat_case, normwise, o_euler, o_heat, o_plugins, o_swept,
+                         oracle_stages)
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.mark.parametrize("fe", [True, False], ids=["fe", "rk2"])
+@pytest.mark.parametrize("swept,b", [(False, 8), (True, 8), (True, 16), (True, 32)])
+def test_heat_matches_oracle(device, fe, swept, b):
+    n, steps = 64, 40
+    s = 1 if fe else 2
+    ic, g, step = heat_case(n, fe, steps, seed=1)
+    sol = gpu_solve("heat", ic, g, 1, s, b, swept=swept, exact=True)
+    want = oracle_stages(step, ic, s, s * steps)
+    assert normwise(sol.final_state, want) <= TOL
+    assert np.array_equal(sol.final_state, want)
+
+
+@pytest.mark.parametrize("swept,b", [(False, 8), (True, 8), (True, 16), (True, 32)])
+def test_euler_matches_oracle(device, swept, b):
+    n, steps = 64, 12
+    ic, g, step = euler_case(n, steps, eps=5.0)
+    sol = gpu_solve("euler", ic, g, 2, 2, b, swept=swept, exact=True)
+    want = oracle_stages(step, ic, 2, 2 * steps)
+    assert normwise(sol.final_state, want) <= TOL
+    assert np.array_equal(sol.final_state, want)
+
+
+def test_euler_one_stage_parity_with_reference_pow_along_trajectory(device):
+    """Each GPU stage against one stage of the faithful numpy path (the C
+    library's pow for squares, euler.py:194, 239) applied to the GPU's own
+    previous levels."""
+    from pysweep_amd import device as dev
+    from pysweep_amd.equations import path as eq_path
+    from pysweep_amd.plugins import load_cpu_module
+    n, stages = 48, 12
+    ic, g, _ = euler_case(n, stages // 2, eps=4.0)
+    run = dev.StandardRun(eq_path("euler.cu"), 4, 2, 2, n, n, T=stages + 2)
+    cpu = load_cpu_module(eq_path("euler.py"))
+    cpu.set_globals(*g)
+    run.set_globals(cpu, g)
+    run.upload_ic(ic)
+    levels = [run.level(0), run.level(1)]
+    for gts in range(1, stages + 1):
+        run.run(gts, gts + 1)
+        levels.append(run.level(gts + 1))
+    dx, dt = g[3], g[2]
+    for gts in range(1, stages + 1):
+        A = gts  # level read
+        want = o_euler.step_py(levels[A], levels[A - 1], gts, 1.4, dt, dx, dx, sq=o_euler.sq_pow)
+        assert normwise(levels[A + 1], want) <= TOL, f"stage {gts}"
+    run.close()
+
+
+def _swept_vs_standard(name, ic, g, k, s, b, steps):
+    a = gpu_solve(name, ic, g, k, s, b, swept=True, exact=True)
+    c = gpu_solve(name, ic, g, k, s, b, swept=False, exact=True)
+    return a.final_state, c.final_state
+
+
+@pytest.mark.parametrize("b", [8, 16, 32])
+def test_heat_swept_equals_standard_bitwise(device, b):
+    ic, g, _ = heat_case(128, True, 100, seed=2)
+    a, c = _swept_vs_standard("heat", ic, g, 1, 1, b, 100)
+    assert np.array_equal(a, c)
+
+
+@pytest.mark.parametrize("b", [8, 16, 32])
+def test_euler_swept_equals_standard_bitwise(device, b):
+    ic, g, _ = euler_case(128, 30, eps=3.0, xc=1.0, yc=-2.0)
+    a, c = _swept_vs_standard("euler", ic, g, 2, 2, b, 30)
+    assert np.array_equal(a, c)
+
+
+def test_non_square_grid(device):
+    """nx != ny (the batched workloads use non-square grids)."""
+    nx, ny, steps = 64, 96, 6
+    dx = 10.0 / nx
+    dt = 0.1 * dx
+    X, Y = np.meshgrid(o_euler.grid(nx), np.linspace(-5, 5, ny, endpoint=False), indexing="ij")
+    ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=3.0)
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    sol = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True)
+    std = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
+    want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
+    assert np.array_equal(sol.final_state, want)
+    assert np.array_equal(std.final_state, want)
+
+
+# ---------------------------------------------------------------- known answers
+
+def _kat_globals(steps, scheme=None):
+    g = [0.0, steps * 0.1, 0.1, 0.1, 0.1]
+    return g + ([scheme] if scheme is not None else [])
+
+
+def test_known_answer_simple_one(device):
+    """testSimpleOne (tests.py:255-286): cell == i at step i, every step."""
+    n, steps = 64, 60
+    sol = gpu_solve("example", np.zeros((1, n, n)), _kat_globals(steps, True), 1, 1, 8, swept=True,
+                    exact=False, history=True)
+    for i in range(sol.arrayShape[0]):
+        assert np.all(sol.data[i] == i), i
+
+
+def test_known_answer_simple_two(device):
+    """testSimpleTwo (tests.py:288-320): k=2, s=2: cell == 2i at step i."""
+    n, steps = 64, 60
+    sol = gpu_solve("example", np.zeros((1, n, n)), _kat_globals(steps, False), 2, 2, 16, swept=True,
+                    exact=False, history=True)
+    for i in range(sol.arrayShape[0]):
+        assert np.all(sol.data[i] == 2 * i), i
+
+
+def test_known_answer_checker_one(device):
+    """testCheckerOne (tests.py:322-366)."""
+    n, steps = 64, 40
+    ic = o_plugins.checker_ic(1, n, n)
+    sol = gpu_solve("checker", ic, _kat_globals(steps, True), 1, 1, 8, swept=True, exact=False, history=True)
+    p1, p2 = ic[0], np.roll(ic[0], 1, axis=0)
+    for i in range(sol.arrayShape[0]):
+        assert np.array_equal(sol.data[i, 0], p1 if i % 2 == 0 else p2), i
+
+
+def test_known_answer_checker_two(device):
+    """testCheckerTwo (tests.py:369-408)."""
+    n, steps = 64, 40
+    ic = o_plugins.checker_ic(1, n, n)
+    sol = gpu_solve("checker", ic, _kat_globals(steps, False), 2, 2, 16, swept=True, exact=False, history=True)
+    for i in range(sol.arrayShape[0]):
+        assert np.array_equal(sol.data[i, 0], ic[0]), i
+
+
+@pytest.mark.parametrize("swept", [True, False])
+def test_known_answer_half(device, swept):
+    """testHalf (tests.py:222-253): rtol=atol=9e-15 against the in-test
+    restatement, at every recorded step."""
+    n, steps = 48, 30
+    ic = o_plugins.half_ic(n, n)
+    sol = gpu_solve("half", ic, _kat_globals(steps), 1, 2, 8, swept=swept, exact=False, history=True)
+    a = ic[0].copy()
+    for i in range(1, sol.arrayShape[0]):
+        a = o_plugins.half_brute_force(a, 1)
+        assert np.allclose(sol.data[i, 0], a, rtol=9e-15, atol=9e-15), i
+
+
+# ---------------------------------------------------------------- configurations
+
+def test_config_heat_384_swept_cpu_case(device):
+    """Heat 384^2, b=8, forward Euler, 100 steps with the reference's
+    quantization (99 stages): oracle bitwise, exact discrete Fourier solution
+    and the analytic decay of the sin(2 pi x) sin(2 pi y) mode."""
+    n, steps = 384, 100
+    ic, g, step = heat_case(n, True, steps, seed=0)
+    sol = gpu_solve("heat", ic, g, 1, 1, 8, swept=True, exact=False)
+    assert sol.times.stages == 99 and sol.times.G == 32
+    want = oracle_stages(step, ic, 1, 99)
+    assert np.array_equal(sol.final_state, want)
+    c = o_heat.courant_py(1.0, g[2], g[3])
+    four = o_heat.fourier_solution(ic[0], 99, c, True)
+    assert np.abs(sol.final_state[0] - four).max() <= 1e-12 * np.abs(four).max()
+    x = o_heat.grid(n)
+    mode = np.outer(np.sin(2 * np.pi * x), np.sin(2 * np.pi * x))
+    amp = (sol.final_state[0] * mode).sum() / (mode * mode).sum()
+    amp0 = (ic[0] * mode).sum() / (mode * mode).sum()
+    t = 99 * g[2]
+    assert abs(amp / amp0 - np.exp(-8 * np.pi ** 2 * t)) < 1e-4
+
+
+def test_config_euler_1024_full_size(device):
+    """Euler 1024^2, b=16, 500 steps (exact): swept == standard bitwise,
+    conservation of the four integrals, and the vortex error of the same
+    size as the oracle's on a coarser grid."""
+    n, steps = 1024, 500
+    ic, g, _ = euler_case(n, steps)
+    sw = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True)
+    st = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
+    assert np.array_equal(sw.final_state, st.final_state)
+    for v in range(4):
+        assert abs(sw.final_state[v].sum() - ic[v].sum()) <= 1e-11 * np.abs(ic[v]).sum()
+    X, Y = np.meshgrid(o_euler.grid(n), o_euler.grid(n), indexing="ij")
+    ex = o_euler.vortex(X, Y, steps * g[2], 1.4)
+    err = np.abs(sw.final_state[0] - ex[0]).max()
+    assert err < 0.05
