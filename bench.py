@@ -1,0 +1,179 @@
+"""Benchmark: 2-D Euler GCUPS, swept vs standard, on one MI355X.
+
+Workload (the reference's GPU configuration): isentropic vortex (eps=5 at the
+origin, free stream (1,1)), periodic [-5,5]^2, nx = ny = 1024, gamma = 1.4,
+dt = 0.1*dx, blocksize 16, operating_points 2, intermediate_steps 2,
+500 full steps (1000 RK2 stages), float64, state resident in HBM.
+
+One bench step = one complete solve of that workload.  ``value`` is the
+swept solve's GCUPS = nx*ny*500 / (seconds per solve) / 1e9; the standard
+kernel's GCUPS on the same inputs is reported beside it.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line (rank 0).  ``roofline`` is for the dominant swept
+kernel (the largest total launch time), with its average launch duration
+measured by HIP events around every launch on the work stream; ``traffic``
+comes from the committed rocprofv3 counter summary when present.
+``cpu_baseline`` times the numpy oracle (the reference's numpy step order)
+on a bounded sample of the same grid.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "Euler 2D GCUPS swept vs standard, 1 GPU; served-solve p50/p99 latency"
+FLOPS_PER_CELL_STAGE = 626          # euler.cu:22-244 counted per stage (1252 per RK2 step)
+FP64_PEAK_TFLOPS = 78.6             # MI355X FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_euler1024_b16.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--b", type=int, default=16)
+    ap.add_argument("--nsteps", type=int, default=500)
+    ap.add_argument("--cpu-stages", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["swept", "standard", "both"], default="both")
+    return ap.parse_args()
+
+
+def setup(n, b, nsteps):
+    import numpy as np
+
+    from oracle import euler as o_euler  # initial condition generator only
+    from pysweep_amd import device, hip, plugins, schedule
+    from pysweep_amd.equations import path as eq_path
+
+    hip.init(0)
+    dx = 10.0 / n
+    dt = 0.1 * dx
+    ic = o_euler.initial_condition(n)
+    g = [0.0, nsteps * dt, dt, dx, dx, 1.4]
+    cpu = plugins.load_cpu_module(eq_path("euler.py"))
+    cpu.set_globals(*g)
+    sw = device.SweptRun(eq_path("euler.cu"), 4, 2, 2, n, n, b)
+    sw.set_globals(cpu, g)
+    sw.keep_ic_on_device(ic)
+    st = device.StandardRun(eq_path("euler.cu"), 4, 2, 2, n, n)
+    st.set_globals(cpu, g)
+    st.keep_ic_on_device(ic)
+    times = schedule.swept_times(nsteps, b, 2, 2, exact=True)
+    return np, hip, ic, g, sw, st, times
+
+
+def time_solves(hip, run_once, reset, K, W):
+    for _ in range(W):
+        reset()
+        run_once(None)
+    hip.synchronize()
+    stats = hip.RunStats()
+    t0 = time.perf_counter()
+    for _ in range(K):
+        reset()
+        run_once(stats)
+    hip.synchronize()
+    return (time.perf_counter() - t0) / K, stats
+
+
+def cpu_baseline(ic, g, stages):
+    """numpy oracle (py order) on the full grid for a few stages."""
+    import numpy as np
+
+    from oracle import euler as o_euler
+    dt, dx = g[2], g[3]
+    q, qb = ic.copy(), ic.copy()
+    t0 = time.perf_counter()
+    for gts in range(1, stages + 1):
+        nq = o_euler.step_py(q, qb, gts, 1.4, dt, dx, dx)
+        qb, q = q, nq
+    sec = time.perf_counter() - t0
+    n = ic.shape[1] * ic.shape[2]
+    steps = stages / 2
+    return {"value": n * steps / sec / 1e9, "unit": "GCUPS", "cores": 1, "kind": "port",
+            "sample": f"{stages} RK2 stages ({steps:g} steps) of the {ic.shape[1]}x{ic.shape[2]} grid with the "
+                      f"vectorized numpy oracle in the reference's numpy step order, {sec:.1f} s",
+            "seconds": sec, "np_version": np.__version__}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
+              "(the x-strip multi-GPU decomposition is design only)", file=sys.stderr)
+        sys.exit(2)
+    np, hip, ic, g, sw, st, times = setup(a.n, a.b, a.nsteps)
+    from pysweep_amd import schedule
+    cells = a.n * a.n
+    res = {}
+    if a.only in ("swept", "both"):
+        sec, stats = time_solves(hip, lambda s: sw.run(times, stats=s), sw.reset, a.steps, a.warmup)
+        res["swept"] = (sec, stats)
+    if a.only in ("standard", "both"):
+        stages = 2 * a.nsteps
+        sec, stats = time_solves(hip, lambda s: st.run(1, stages + 1, stats=s), st.reset, a.steps, a.warmup)
+        res["standard"] = (sec, stats)
+
+    out = {"metric": METRIC, "unit": "GCUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
+    gc = {k: cells * a.nsteps / v[0] / 1e9 for k, v in res.items()}
+    head = "swept" if "swept" in res else "standard"
+    out["value"] = gc[head]
+    out["ms_per_step"] = res[head][0] * 1e3
+    out["config"] = {"workload": f"euler-vortex-{a.n}x{a.n}-b{a.b}-{a.nsteps}steps-{head}", "nx": a.n, "ny": a.n,
+                     "blocksize": a.b, "operating_points": 2, "intermediate_steps": 2, "steps_per_solve": a.nsteps,
+                     "method": head, "G": times.G, "swept_stages": times.stages, "tail_stages": times.tail}
+    if "standard" in gc:
+        out["standard_gcups"] = gc["standard"]
+        out["standard_ms_per_solve"] = res["standard"][0] * 1e3
+    if "swept" in gc and "standard" in gc:
+        out["swept_over_standard"] = gc["swept"] / gc["standard"]
+
+    # roofline of the dominant kernel of the headline method
+    stats = res[head][1].as_dict()
+    steps_timed = a.steps
+    kern = max(stats, key=lambda k: stats[k]["ms"])
+    P = times.P if head == "swept" else None
+    tiles = (a.n // a.b) ** 2
+    if kern == "Standard":
+        cs = cells
+    else:
+        cs = tiles * schedule.phase_cell_stages(kern, a.b, 2, P)
+    avg_s = stats[kern]["avg_ms"] / 1e3
+    achieved = cs * FLOPS_PER_CELL_STAGE / avg_s / 1e12
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get(kern, {}).get("bytes_per_launch")
+    out["roofline"] = {"bound": "mfma", "pipe": "fp64-valu", "kernel": kern, "achieved": achieved,
+                       "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
+                       "traffic": traffic, "avg_launch_ms": stats[kern]["avg_ms"],
+                       "cell_stages_per_launch": cs, "flops_per_cell_stage": FLOPS_PER_CELL_STAGE,
+                       "launches_timed": stats[kern]["launches"]}
+    out["kernel_ms_per_solve"] = {k: v["ms"] / steps_timed for k, v in stats.items() if v["launches"]}
+    for m in res:
+        out[f"{m}_kernel_ms_per_solve"] = {k: v["ms"] / steps_timed for k, v in res[m][1].as_dict().items()
+                                           if v["launches"]}
+    # parity of this run: swept == standard bitwise on the final level
+    if "swept" in res and "standard" in res:
+        fin_sw = sw.level(times.stages + times.tail + 1)
+        fin_st = st.level(2 * a.nsteps + 1)
+        out["swept_equals_standard_bitwise"] = bool(np.array_equal(fin_sw, fin_st))
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(ic, g, a.cpu_stages)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

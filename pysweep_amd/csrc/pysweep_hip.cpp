@@ -59,8 +59,75 @@ int launch(ps_function f, unsigned gx, unsigned threads, void **args)
     return 0;
 }
 
+// Optional per-kernel timing: an event pair around every launch, resolved
+// after the run (ps_run_stats in the header).
+struct Timer {
+    ps_run_stats *stats = nullptr;
+    std::vector<hipEvent_t> ev;   // 2 per launch
+    std::vector<int> kind;
+    size_t used = 0;
+};
+
+Timer g_timer;
+
+int timer_begin(ps_run_stats *stats)
+{
+    g_timer.stats = stats;
+    g_timer.used = 0;
+    g_timer.kind.clear();
+    return 0;
+}
+
+int timer_mark(bool start, int kind)
+{
+    if (!g_timer.stats) return 0;
+    if (start) {
+        if (g_timer.used + 2 > g_timer.ev.size()) {
+            for (int i = 0; i < 64; ++i) {
+                hipEvent_t e;
+                PS_HIP(hipEventCreate(&e));
+                g_timer.ev.push_back(e);
+            }
+        }
+        g_timer.kind.push_back(kind);
+    }
+    PS_HIP(hipEventRecord(g_timer.ev[g_timer.used++], g_stream));
+    return 0;
+}
+
+int timer_end()
+{
+    ps_run_stats *st = g_timer.stats;
+    g_timer.stats = nullptr;
+    if (!st) return 0;
+    PS_HIP(hipStreamSynchronize(g_stream));
+    for (size_t i = 0; i < g_timer.kind.size(); ++i) {
+        float ms = 0.f;
+        PS_HIP(hipEventElapsedTime(&ms, g_timer.ev[2 * i], g_timer.ev[2 * i + 1]));
+        const int k = g_timer.kind[i];
+        st->ms[k] += ms;
+        st->launches[k] += 1;
+    }
+    return 0;
+}
+
+int launch_kind(ps_function f, unsigned gx, unsigned threads, void **args, int kind)
+{
+    int rc = timer_mark(true, kind);
+    if (rc) return rc;
+    if ((rc = launch(f, gx, threads, args))) return rc;
+    return timer_mark(false, kind);
+}
+
+__global__ void ps_copy_kernel(double *__restrict__ dst, const double *__restrict__ src, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
 // One phase launch: K(state, gts, A0, grid).
-int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xoff)
+int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xoff, int kind)
 {
     if (!f) return fail("swept plan has a null phase kernel");
     ps_grid g;
@@ -72,7 +139,8 @@ int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xof
     g.tiles_y = p->ny / p->b;
     void *state = p->state;
     void *args[] = {&state, &gts, &A0, &g};
-    return launch(f, static_cast<unsigned>(g.tiles_x * g.tiles_y), static_cast<unsigned>(p->threads), args);
+    return launch_kind(f, static_cast<unsigned>(g.tiles_x * g.tiles_y), static_cast<unsigned>(p->threads), args,
+                       kind);
 }
 
 }  // namespace
@@ -181,7 +249,10 @@ int ps_module_get_global(ps_module module, const char *name, void **dptr, size_t
     hipDeviceptr_t p = nullptr;
     size_t n = 0;
     hipError_t e = hipModuleGetGlobal(&p, &n, static_cast<hipModule_t>(module), name);
-    if (e != hipSuccess) return fail(std::string("no device global named '") + name + "' in the module");
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // a failed lookup must not poison later calls
+        return fail(std::string("no device global named '") + name + "' in the module");
+    }
     *dptr = p;
     *bytes = n;
     return 0;
@@ -191,7 +262,10 @@ int ps_module_get_function(ps_module module, const char *name, ps_function *func
 {
     hipFunction_t f = nullptr;
     hipError_t e = hipModuleGetFunction(&f, static_cast<hipModule_t>(module), name);
-    if (e != hipSuccess) return fail(std::string("no kernel named '") + name + "' in the module");
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // a failed lookup must not poison later calls
+        return fail(std::string("no kernel named '") + name + "' in the module");
+    }
     *function = f;
     return 0;
 }
@@ -297,9 +371,8 @@ int ps_event_destroy(ps_event event)
     return 0;
 }
 
-int ps_swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
+static int swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
 {
-    PS_NEED_INIT();
     if (!p || !p->state) return fail("swept plan without state");
     const int b = p->b, k = p->ops, s = p->its;
     if (b <= 0 || k <= 0 || s <= 0 || b % (2 * k) != 0) return fail("invalid block size / operating points");
@@ -314,29 +387,43 @@ int ps_swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
         int rc = 0;
         if (c == 0) {
             // FirstPrism: stages 1..P
-            if ((rc = launch_phase(p, p->up, 1, base, 0))) return rc;
-            if ((rc = launch_phase(p, p->ybridge, 1, base, 0))) return rc;
+            if ((rc = launch_phase(p, p->up, 1, base, 0, PS_K_UP))) return rc;
+            if ((rc = launch_phase(p, p->ybridge, 1, base, 0, PS_K_YBRIDGE))) return rc;
         } else if (c <= G) {
             // UpPrism c: stages (c-1)P+1 .. (c+1)P
             const int gts = (c - 1) * P + 1;
             const int A = base + (c - 1) * P;
-            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff))) return rc;
-            if ((rc = launch_phase(p, p->octahedron, gts, A, xoff))) return rc;
-            if ((rc = launch_phase(p, p->ybridge, gts + P, A + P, xoff))) return rc;
+            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff, PS_K_XBRIDGE))) return rc;
+            if ((rc = launch_phase(p, p->octahedron, gts, A, xoff, PS_K_OCTAHEDRON))) return rc;
+            if ((rc = launch_phase(p, p->ybridge, gts + P, A + P, xoff, PS_K_YBRIDGE))) return rc;
         } else {
             // LastPrism: stages GP+1 .. (G+1)P
             const int gts = G * P + 1;
             const int A = base + G * P;
-            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff))) return rc;
-            if ((rc = launch_phase(p, p->down, gts, A, xoff))) return rc;
+            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff, PS_K_XBRIDGE))) return rc;
+            if ((rc = launch_phase(p, p->down, gts, A, xoff, PS_K_DOWN))) return rc;
         }
     }
     return 0;
 }
 
-int ps_standard_run(const ps_standard_plan *p, int gts_begin, int gts_end)
+int ps_swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
 {
     PS_NEED_INIT();
+    return swept_run(p, G, c_begin, c_end);
+}
+
+int ps_swept_run_timed(const ps_swept_plan *p, int G, int c_begin, int c_end, ps_run_stats *stats)
+{
+    PS_NEED_INIT();
+    timer_begin(stats);
+    int rc = swept_run(p, G, c_begin, c_end);
+    int rc2 = timer_end();
+    return rc ? rc : rc2;
+}
+
+static int standard_run(const ps_standard_plan *p, int gts_begin, int gts_end)
+{
     if (!p || !p->state || !p->standard) return fail("incomplete standard plan");
     if (p->T < p->its + 1) return fail("standard ring must hold at least its+1 levels");
     if (p->bx <= 0 || p->by <= 0 || p->bx > p->nx || p->by > p->ny) return fail("invalid standard tile");
@@ -351,10 +438,39 @@ int ps_standard_run(const ps_standard_plan *p, int gts_begin, int gts_end)
     for (int gts = gts_begin; gts < gts_end; ++gts) {
         int A = p->its - 2 + gts;
         void *args[] = {&state, &gts, &A, &g};
-        int rc = launch(p->standard, static_cast<unsigned>(g.tiles_x * g.tiles_y),
-                        static_cast<unsigned>(p->threads), args);
+        int rc = launch_kind(p->standard, static_cast<unsigned>(g.tiles_x * g.tiles_y),
+                             static_cast<unsigned>(p->threads), args, PS_K_STANDARD);
         if (rc) return rc;
     }
+    return 0;
+}
+
+int ps_standard_run(const ps_standard_plan *p, int gts_begin, int gts_end)
+{
+    PS_NEED_INIT();
+    return standard_run(p, gts_begin, gts_end);
+}
+
+int ps_standard_run_timed(const ps_standard_plan *p, int gts_begin, int gts_end, ps_run_stats *stats)
+{
+    PS_NEED_INIT();
+    timer_begin(stats);
+    int rc = standard_run(p, gts_begin, gts_end);
+    int rc2 = timer_end();
+    return rc ? rc : rc2;
+}
+
+int ps_copy_f64(void *dst, const void *src, size_t n)
+{
+    PS_NEED_INIT();
+    const unsigned threads = 256;
+    size_t blocks = (n + threads - 1) / threads;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) return 0;
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(ps_copy_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, g_stream,
+                       static_cast<double *>(dst), static_cast<const double *>(src), n);
+    PS_HIP(hipGetLastError());
     return 0;
 }
 

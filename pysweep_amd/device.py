@@ -35,6 +35,18 @@ class _Run:
         for A in range(self.its):
             self.ring.upload(ic, (A % self.T) * self.level_bytes)
 
+    def keep_ic_on_device(self, ic):
+        """Device copy of the initial condition for stream-ordered resets."""
+        ic = numpy.ascontiguousarray(ic, dtype=numpy.float64)
+        self.ic_dev = hip.DeviceArray(ic.nbytes)
+        self.ic_dev.upload(ic)
+
+    def reset(self):
+        """Levels 0..s-1 <- initial condition, on the work stream (no host copy)."""
+        n = self.level_bytes // 8
+        for A in range(self.its):
+            hip.copy_f64(self.ring.ptr + (A % self.T) * self.level_bytes, self.ic_dev.ptr, n)
+
     def level(self, A, out=None):
         out = numpy.empty((self.nv, self.nx, self.ny)) if out is None else out
         return self.ring.download(out, (A % self.T) * self.level_bytes)
@@ -55,8 +67,12 @@ class StandardRun(_Run):
                                      by=self.spec.by, threads=self.spec.threads,
                                      standard=mod.get_function("Standard"))
 
-    def run(self, g_begin, g_end, on_level=None):
-        """Stages g_begin..g_end-1; ``on_level(A)`` after each full step."""
+    def run(self, g_begin, g_end, on_level=None, stats=None):
+        """Stages g_begin..g_end-1; ``on_level(A)`` after each full step;
+        ``stats`` (hip.RunStats) accumulates per-launch event timings."""
+        if stats is not None:
+            hip.standard_run_timed(self.plan, g_begin, g_end, stats)
+            return
         if on_level is None:
             hip.standard_run(self.plan, g_begin, g_end)
             return
@@ -93,14 +109,18 @@ class SweptRun(_Run):
             self.tail = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=sspec.bx,
                                          by=sspec.by, threads=sspec.threads, standard=smod.get_function("Standard"))
 
-    def run(self, times, on_level=None):
+    def run(self, times, on_level=None, stats=None):
         """Run the swept cycles of ``times`` (schedule.SweptTimes) and then
         its tail stages; ``on_level(A)`` for every full-step level as soon as
-        it is complete.  Returns the final absolute level."""
+        it is complete; ``stats`` (hip.RunStats) accumulates per-launch event
+        timings.  Returns the final absolute level."""
         G, P, s = times.G, self.P, self.its
         done = s - 1
         if G >= 0:
-            if on_level is None:
+            if stats is not None and on_level is None:
+                hip.swept_run_timed(self.plan, G, 0, G + 2, stats)
+                done = schedule.complete_after_cycle(G + 1, G, P, s)
+            elif on_level is None:
                 hip.swept_run(self.plan, G, 0, G + 2)
                 done = schedule.complete_after_cycle(G + 1, G, P, s)
             else:
@@ -114,7 +134,9 @@ class SweptRun(_Run):
             if self.tail is None:
                 raise RuntimeError("exact-step tail needs the standard kernel")
             g0 = done - (s - 1) + 1
-            for g in range(g0, g0 + times.tail):
+            if stats is not None and on_level is None:
+                hip.standard_run_timed(self.tail, g0, g0 + times.tail, stats)
+            for g in (range(g0, g0 + times.tail) if on_level is not None or stats is None else ()):
                 hip.standard_run(self.tail, g, g + 1)
                 A = s - 1 + g
                 if on_level is not None and (A - (s - 1)) % s == 0:

@@ -45,6 +45,19 @@ class StandardPlan(ctypes.Structure):
                 ("bx", c_int), ("by", c_int), ("threads", c_int), ("standard", c_void_p)]
 
 
+class RunStats(ctypes.Structure):
+    """Per-kernel launch time totals (index: KERNEL_KINDS)."""
+    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_longlong * 6)]
+
+    def as_dict(self):
+        return {k: {"ms": self.ms[i], "launches": self.launches[i],
+                    "avg_ms": self.ms[i] / self.launches[i] if self.launches[i] else None}
+                for i, k in enumerate(KERNEL_KINDS)}
+
+
+KERNEL_KINDS = ("UpPyramid", "YBridge", "XBridge", "Octahedron", "DownPyramid", "Standard")
+
+
 def _sig(name, *argtypes, restype=c_int):
     f = getattr(_lib, name)
     f.argtypes = list(argtypes)
@@ -82,6 +95,11 @@ _event_elapsed = _sig("ps_event_elapsed_ms", c_void_p, c_void_p, ctypes.POINTER(
 _event_destroy = _sig("ps_event_destroy", c_void_p)
 _swept_run = _sig("ps_swept_run", ctypes.POINTER(SweptPlan), c_int, c_int, c_int)
 _standard_run = _sig("ps_standard_run", ctypes.POINTER(StandardPlan), c_int, c_int)
+_swept_run_timed = _sig("ps_swept_run_timed", ctypes.POINTER(SweptPlan), c_int, c_int, c_int,
+                        ctypes.POINTER(RunStats))
+_standard_run_timed = _sig("ps_standard_run_timed", ctypes.POINTER(StandardPlan), c_int, c_int,
+                           ctypes.POINTER(RunStats))
+_copy_f64 = _sig("ps_copy_f64", c_void_p, c_void_p, c_size_t)
 _graph_begin = _sig("ps_graph_begin")
 _graph_end = _sig("ps_graph_end", ctypes.POINTER(c_void_p))
 _graph_launch = _sig("ps_graph_launch", c_void_p)
@@ -93,7 +111,8 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_module_get_function", "ps_function_info", "ps_malloc", "ps_free", "ps_memcpy_htod",
            "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_launch", "ps_synchronize",
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_destroy",
-           "ps_swept_run", "ps_standard_run", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
+           "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
+           "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
 
 
@@ -252,6 +271,19 @@ def swept_run(plan, G, c_begin, c_end):
 
 def standard_run(plan, g_begin, g_end):
     check(_standard_run(ctypes.byref(plan), g_begin, g_end), "ps_standard_run")
+
+
+def swept_run_timed(plan, G, c_begin, c_end, stats):
+    check(_swept_run_timed(ctypes.byref(plan), G, c_begin, c_end, ctypes.byref(stats)), "ps_swept_run_timed")
+
+
+def standard_run_timed(plan, g_begin, g_end, stats):
+    check(_standard_run_timed(ctypes.byref(plan), g_begin, g_end, ctypes.byref(stats)), "ps_standard_run_timed")
+
+
+def copy_f64(dst, src, n):
+    """Stream-ordered device copy of n doubles by the library's copy kernel."""
+    check(_copy_f64(dst, src, n), "ps_copy_f64")
 
 
 class Event:

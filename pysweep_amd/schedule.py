@@ -74,3 +74,36 @@ def full_step_levels(s, lo, hi):
     if r:
         first += s - r
     return list(range(first, hi + 1, s))
+
+
+PHASES = ("UpPyramid", "YBridge", "XBridge", "Octahedron", "DownPyramid")
+
+
+def phase_set(kind, r, b, k, P):
+    """Active rectangle (x0, x1, y0, y1) of tile-local level r of a phase
+    (core/block.py:123-164; core/pysweep.cu:64-249)."""
+    h = b // 2
+    if kind == "Octahedron":
+        if r <= P:
+            return (h - r * k, h + r * k, h - r * k, h + r * k)
+        q = r - P
+        return (q * k, b - q * k, q * k, b - q * k)
+    if kind == "UpPyramid":
+        return (r * k, b - r * k, r * k, b - r * k)
+    if kind == "DownPyramid":
+        return (h - r * k, h + r * k, h - r * k, h + r * k)
+    if kind == "YBridge":
+        return (r * k, b - r * k, h - r * k, h + r * k)
+    if kind == "XBridge":
+        return (h - r * k, h + r * k, r * k, b - r * k)
+    raise ValueError(kind)
+
+
+def phase_cell_stages(kind, b, k, P):
+    """Cell-stages one tile of a phase computes (sum of its set sizes)."""
+    L = 2 * P if kind == "Octahedron" else P
+    tot = 0
+    for r in range(1, L + 1):
+        x0, x1, y0, y1 = phase_set(kind, r, b, k, P)
+        tot += (x1 - x0) * (y1 - y0)
+    return tot
