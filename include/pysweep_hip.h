@@ -164,6 +164,43 @@ typedef struct ps_run_stats {
 int ps_swept_run_timed(const ps_swept_plan *plan, int G, int c_begin, int c_end, ps_run_stats *stats);
 int ps_standard_run_timed(const ps_standard_plan *plan, int gts_begin, int gts_end, ps_run_stats *stats);
 
+/* ---------------------------------------------------------------------- */
+/* Instance tables: many problems of one plugin class per launch.          */
+/* (No counterpart in the reference, which solves one problem per run.)    */
+/* ---------------------------------------------------------------------- */
+
+/* One problem (layout shared with the kernels). */
+typedef struct ps_inst {
+    void *ring;          /* state ring [T][nv][nx][ny] of the class's type  */
+    const void *consts;  /* the plugin's constants for this problem         */
+    int nx, ny, T;
+    int tiles_x, tiles_y;    /* swept tiles (b x b)                          */
+    int stiles_x, stiles_y;  /* standard tiles                               */
+    int G;               /* UpPrism cycles, -1 if none                       */
+    int tail;            /* standard stages after LastPrism                  */
+    int c;               /* round counter: 0 FirstPrism, 1..G UpPrism,        */
+                         /* G+1 LastPrism, G+2.. tail stages                 */
+    int active;
+    int pad_;
+} ps_inst;
+
+typedef struct ps_multi_plan {
+    void *insts;        /* device ps_inst[ninst]                            */
+    void *prefix;       /* device int[ninst+1]: prefix sums of swept tiles  */
+    void *sprefix;      /* device int[ninst+1]: prefix sums of std tiles    */
+    int ninst;
+    int total_tiles, total_stiles;
+    int grid, sgrid;    /* persistent workgroups; 0 = one per tile          */
+    int threads, sthreads;
+    ps_function up, ybridge, xbridge, octahedron, down, standard, advance;
+} ps_multi_plan;
+
+/* Run `rounds` rounds: each round launches UpPyramidMulti, XBridgeMulti,
+ * OctahedronMulti, DownPyramidMulti, YBridgeMulti, StandardMulti and
+ * ps_advance; every problem executes the phases of its own cycle c. */
+int ps_multi_run(const ps_multi_plan *plan, int rounds);
+int ps_multi_run_timed(const ps_multi_plan *plan, int rounds, ps_run_stats *stats);
+
 /* Device-to-device copy of n doubles by a plain copy kernel on the work
  * stream (ring resets, and the reference copy of known size that calibrates
  * HBM byte counters). */

@@ -1,0 +1,105 @@
+"""Autotuner for instance-table classes: block size, LDS window, workgroup
+size and persistent blocks per CU.
+
+``Tuner.tune`` runs a small sample batch with each candidate launch shape
+and keeps the fastest (cell-steps per second); every candidate computes the
+same per-cell arithmetic, so the choice never changes results (checked by
+the GPU tests).  Choices persist in a JSON file next to the code-object
+cache; a class without a record uses the measured defaults below.
+"""
+import json
+import os
+import time
+
+from pysweep_amd import compiler
+
+DEFAULT_PATH = os.path.join(compiler.CACHE, "tuning.json")
+
+#: defaults from MI355X measurements of the Euler vortex (1024^2, 500 steps):
+#: b=16 with a 5-level window ran 3.7 GCUPS, b=32 with 4 levels and 512
+#: threads 4.1 GCUPS (single-problem launches).
+DEFAULTS = {
+    ("euler", "fp64"): {"b": 16, "grid_per_cu": 0},
+    ("euler", "fp32"): {"b": 16, "grid_per_cu": 0},
+    ("heat", "fp64"): {"b": 32, "grid_per_cu": 0},
+    ("heat", "fp32"): {"b": 32, "grid_per_cu": 0},
+}
+
+CANDIDATES = {
+    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "grid_per_cu": 8}, {"b": 32, "grid_per_cu": 4}],
+    "heat": [{"b": 16}, {"b": 32}, {"b": 32, "grid_per_cu": 8}, {"b": 64}],
+}
+
+
+def _key(plugin, precision, its):
+    return f"{plugin}/{precision}/its{its}"
+
+
+class Tuner:
+    def __init__(self, path=DEFAULT_PATH):
+        self.path = path
+        self.records = {}
+        if path and os.path.exists(path):
+            with open(path) as f:
+                self.records = json.load(f)
+
+    def record(self, plugin, precision, its):
+        r = self.records.get(_key(plugin, precision, its))
+        if r is None:
+            r = dict(DEFAULTS.get((plugin, precision), {"b": 16, "grid_per_cu": 0}))
+        return r
+
+    def block_size(self, plugin, precision, its, nx, ny):
+        b = self.record(plugin, precision, its).get("b", 16)
+        for cand in (b, 32, 16, 8):
+            if nx % cand == 0 and ny % cand == 0 and cand <= min(nx, ny):
+                return cand
+        raise ValueError(f"grid {nx}x{ny} has no supported block size")
+
+    def config(self, plugin, precision, its, b):
+        r = self.record(plugin, precision, its)
+        out = {"grid_per_cu": r.get("grid_per_cu", 0)}
+        if r.get("b") == b:
+            for k in ("slots", "threads"):
+                if k in r:
+                    out[k] = r[k]
+        return out
+
+    def tune(self, plugin, precision, its, make_problems, candidates=None, save=True):
+        """Time each candidate on ``make_problems()`` (a list of Problems);
+        return and store the fastest."""
+        from pysweep_amd import batch
+        trials = []
+        for cand in candidates or CANDIDATES[plugin]:
+            probs = make_problems()
+            for p in probs:
+                p.b = cand["b"]
+
+            class _Fixed:
+                def block_size(self_, *a):
+                    return cand["b"]
+
+                def config(self_, *a):
+                    return {k: v for k, v in cand.items() if k != "b"}
+
+            eng = batch.Engine(tuner=_Fixed())
+            for p in probs:
+                eng.submit(p, arrival=0.0)
+            eng.step()  # admit + compile + first round (not timed)
+            t0 = time.perf_counter()
+            eng.run()
+            sec = time.perf_counter() - t0
+            cells = 0
+            for t in eng.finished:
+                nv, nx, ny = t.problem.ic.shape
+                cells += nx * ny * (t.times.stages + t.times.tail) / t.problem.its
+            trials.append({**cand, "seconds": sec, "gcups": cells / sec / 1e9})
+        best = max(trials, key=lambda r: r["gcups"])
+        rec = {k: v for k, v in best.items() if k in ("b", "slots", "threads", "grid_per_cu")}
+        rec["trials"] = trials
+        self.records[_key(plugin, precision, its)] = rec
+        if save and self.path:
+            os.makedirs(os.path.dirname(self.path), exist_ok=True)
+            with open(self.path, "w") as f:
+                json.dump(self.records, f, indent=1)
+        return rec

@@ -1,0 +1,369 @@
+"""Many problems on one GPU: batched launches, continuous admission, graphs.
+
+Problems of the same plugin, state precision and block size form a class.
+A class owns one instance-table module of each kernel kind (the swept
+phases and the standard stage, compiled with per-instance constant blocks)
+and a table ``ps_inst[]`` on the device.  One *round* (``ps_multi_run``)
+launches every phase kernel once over all tiles of all problems in the
+table; each problem executes the phases of its own cycle (FirstPrism,
+UpPrism, LastPrism or an exact-step tail stage), so problems admitted at
+different times, with different grids and step counts, share launches.
+
+``Engine`` keeps the state of every admitted problem resident in HBM, admits
+queued problems while the HBM budget allows, retires finished ones (their
+final level is copied to the host and their memory freed), and replays a
+captured hipGraph of the round while the set of running problems does not
+change.  Results do not depend on what else is running: each tile reads only
+its own problem's ring and constants.
+"""
+import ctypes
+import dataclasses
+import math
+import os
+import time
+
+import numpy
+
+from pysweep_amd import hip, kernels, plugins, schedule
+from pysweep_amd.equations import path as eq_path
+
+PLUGINS = {
+    # name: (python half, device source fp64, device source fp32, nv, ops, its)
+    "euler": ("euler.py", "euler.cu", "euler_fp32.cu", 4, 2, 2),
+    "heat": ("heat.py", "heat.cu", "heat_fp32.cu", 1, 1, None),
+}
+
+
+@dataclasses.dataclass(eq=False)
+class Problem:
+    """One solve: plugin name, initial condition (nv, nx, ny), the plugin's
+    globals [t0, tf, dt, dx, dy, ...], intermediate steps ``its`` and state
+    precision ("fp64" or "fp32")."""
+    plugin: str
+    ic: numpy.ndarray
+    globals: list
+    its: int
+    precision: str = "fp64"
+    b: int = None
+    exact: bool = True
+    tag: object = None
+
+
+class _HostConsts:
+    """The plugin's device constants of one problem, written by its
+    set_globals through ``source_mod.get_global`` / ``cuda.memcpy_htod``."""
+
+    def __init__(self, layout):
+        self.layout = layout
+        self.buf = bytearray(layout.size)
+
+    def get_global(self, name):
+        f = self.layout.field(name)
+        return plugins.HostField(self.buf, f.offset, f.size, name), f.size
+
+    def get_function(self, name):
+        raise KeyError(name)
+
+
+def class_specs(plugin, precision, its, b, slots=None, threads=None):
+    """(swept, standard) instance-table module specs of a class."""
+    py, cu64, cu32, nv, ops, _ = PLUGINS[plugin]
+    real = "double" if precision == "fp64" else "float"
+    rb = 8 if precision == "fp64" else 4
+    cu = eq_path(cu64 if precision == "fp64" else cu32)
+    P = b // (2 * ops) - 1
+    spec = kernels.swept_spec(cu, nv, ops, its, b, slots=slots, threads=threads, real=real, multi=True)
+    bx, by = kernels.standard_tile(nv, ops, its, b * 4, b * 4, real_bytes=rb)
+    sspec = kernels.standard_spec(cu, nv, ops, its, max(bx, b), max(by, b), bx=bx, by=by, real=real, multi=True,
+                                  mpss=P)
+    return spec, sspec
+
+
+def shipped_class_specs():
+    """Instance-table modules used by the tests and benchmarks."""
+    out = []
+    for plugin, precision, its, b in (("euler", "fp64", 2, 8), ("euler", "fp64", 2, 16), ("euler", "fp64", 2, 32),
+                                      ("euler", "fp32", 2, 16), ("euler", "fp32", 2, 32),
+                                      ("heat", "fp64", 1, 16), ("heat", "fp64", 1, 32), ("heat", "fp64", 2, 16),
+                                      ("heat", "fp64", 1, 64), ("heat", "fp32", 1, 16), ("heat", "fp32", 1, 32)):
+        out.extend(class_specs(plugin, precision, its, b))
+    return out
+
+
+class PluginClass:
+    """Compiled instance-table modules of one (plugin, precision, b)."""
+
+    def __init__(self, plugin, precision, its, b, slots=None, threads=None, grid_per_cu=0):
+        py, cu64, cu32, nv, ops, default_its = PLUGINS[plugin]
+        self.plugin, self.precision, self.nv, self.ops, self.its, self.b = plugin, precision, nv, ops, its, b
+        self.real = "double" if precision == "fp64" else "float"
+        self.dtype = numpy.float64 if precision == "fp64" else numpy.float32
+        self.rb = numpy.dtype(self.dtype).itemsize
+        self.P = b // (2 * ops) - 1
+        self.spec, self.sspec = class_specs(plugin, precision, its, b, slots, threads)
+        self.layout = self.spec.const_layout()
+        m = kernels.load(self.spec)
+        sm = kernels.load(self.sspec)
+        self.fn = {k: m.get_function(k + "Multi") for k in kernels.SWEPT_KERNELS}
+        self.fn["Standard"] = sm.get_function("StandardMulti")
+        self.fn["advance"] = m.get_function("ps_advance")
+        self.cpu = plugins.load_cpu_module(eq_path(py), name=f"pysweep_batch_{plugin}")
+        self.grid_per_cu = grid_per_cu
+        self.key = (plugin, precision, its, b)
+
+    def ring_bytes(self, nx, ny):
+        return (2 * self.P + self.its) * self.nv * nx * ny * self.rb
+
+
+@dataclasses.dataclass(eq=False)
+class Ticket:
+    problem: Problem
+    cls: PluginClass = None
+    nx: int = 0
+    ny: int = 0
+    times: object = None
+    ring: object = None
+    consts: object = None
+    c: int = 0
+    c_end: int = 0
+    final_level: int = 0
+    result: numpy.ndarray = None
+    arrival: float = 0.0
+    admitted: float = None
+    finished: float = None
+    nbytes: int = 0
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+    @property
+    def done(self):
+        return self.result is not None
+
+
+class _ClassTable:
+    """Device instance table of one class."""
+
+    def __init__(self, cls):
+        self.cls = cls
+        self.running = []
+        self.dirty = True
+        self.insts = self.prefix = self.sprefix = None
+        self.cap = 0
+        self.plan = None
+        self.graph = None
+
+    def rebuild(self, cus):
+        cls = self.cls
+        n = len(self.running)
+        if n == 0:
+            self.plan = None
+            self.graph = None
+            self.dirty = False
+            return
+        if n > self.cap:
+            self.cap = max(8, 2 * n)
+            self.insts = hip.DeviceArray(self.cap * ctypes.sizeof(hip.Inst))
+            self.prefix = hip.DeviceArray((self.cap + 1) * 4)
+            self.sprefix = hip.DeviceArray((self.cap + 1) * 4)
+        arr = (hip.Inst * n)()
+        pre = numpy.zeros(n + 1, dtype=numpy.int32)
+        spre = numpy.zeros(n + 1, dtype=numpy.int32)
+        sbx, sby = cls.sspec.bx, cls.sspec.by
+        for i, t in enumerate(self.running):
+            st = arr[i]
+            st.ring, st.consts = t.ring.ptr, t.consts.ptr
+            st.nx, st.ny, st.T = t.nx, t.ny, 2 * cls.P + cls.its
+            st.tiles_x, st.tiles_y = t.nx // cls.b, t.ny // cls.b
+            st.stiles_x, st.stiles_y = -(-t.nx // sbx), -(-t.ny // sby)
+            st.G, st.tail, st.c, st.active = t.times.G, t.times.tail, t.c, 1
+            pre[i + 1] = pre[i] + st.tiles_x * st.tiles_y
+            spre[i + 1] = spre[i] + (st.stiles_x * st.stiles_y if t.times.tail else 0)
+        hip.check(hip._htod(self.insts.ptr, ctypes.addressof(arr), ctypes.sizeof(arr)), "ps_memcpy_htod")
+        self.prefix.upload(pre)
+        self.sprefix.upload(spre)
+        f = cls.fn
+        grid = cls.grid_per_cu * cus if cls.grid_per_cu else 0
+        self.plan = hip.MultiPlan(insts=self.insts.ptr, prefix=self.prefix.ptr, sprefix=self.sprefix.ptr,
+                                  ninst=n, total_tiles=int(pre[-1]), total_stiles=int(spre[-1]), grid=grid,
+                                  sgrid=grid, threads=cls.spec.threads, sthreads=cls.sspec.threads,
+                                  up=f["UpPyramid"], ybridge=f["YBridge"], xbridge=f["XBridge"],
+                                  octahedron=f["Octahedron"], down=f["DownPyramid"], standard=f["Standard"],
+                                  advance=f["advance"])
+        self.graph = None
+        self.dirty = False
+
+
+class Engine:
+    """Continuous batching of swept solves on one GPU."""
+
+    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None):
+        hip.ensure_init(0)
+        self.hbm_budget = hbm_budget if hbm_budget is not None else 64 << 30
+        self.max_active = max_active
+        self.use_graphs = use_graphs
+        self.cus = cus
+        self.tuner = tuner
+        self.classes = {}
+        self.tables = {}
+        self.queue = []
+        self.running = []
+        self.finished = []
+        self.in_use = 0
+        self.peak_in_use = 0
+        self.rounds = 0
+        self.clock = clock or time.perf_counter
+        self.t0 = self.clock()
+
+    # ---------------------------------------------------------------- API
+    def now(self):
+        return self.clock() - self.t0
+
+    def submit(self, problem, arrival=None):
+        t = Ticket(problem=problem, arrival=self.now() if arrival is None else arrival)
+        self.queue.append(t)
+        return t
+
+    def idle(self):
+        return not self.queue and not self.running
+
+    def run(self, max_rounds=None):
+        """Admit and run until every submitted problem has finished."""
+        r = 0
+        while not self.idle():
+            self.step()
+            r += 1
+            if max_rounds and r >= max_rounds:
+                break
+        return self.finished
+
+    def step(self):
+        """Admit what fits, run one round of every class, retire finished."""
+        self._admit()
+        if not self.running:
+            if self.queue:
+                time.sleep(max(0.0, min(t.arrival for t in self.queue) - self.now()))
+            return
+        for key, tab in self.tables.items():
+            if tab.dirty:
+                tab.rebuild(self.cus)
+            if tab.plan is None:
+                continue
+            if self.use_graphs:
+                if tab.graph is None:
+                    g = hip.Graph()
+                    with g:
+                        hip.multi_run(tab.plan, 1)
+                    tab.graph = g
+                tab.graph.launch()
+            else:
+                hip.multi_run(tab.plan, 1)
+        self.rounds += 1
+        for t in list(self.running):
+            t.c += 1
+            if t.c >= t.c_end:
+                self._retire(t)
+
+    # ---------------------------------------------------------- internals
+    def _class_for(self, p, b):
+        key = (p.plugin, p.precision, p.its, b)
+        cls = self.classes.get(key)
+        if cls is None:
+            cfg = self.tuner.config(p.plugin, p.precision, p.its, b) if self.tuner else {}
+            cls = PluginClass(p.plugin, p.precision, p.its, b, slots=cfg.get("slots"), threads=cfg.get("threads"),
+                              grid_per_cu=cfg.get("grid_per_cu", 0))
+            self.classes[key] = cls
+            self.tables[key] = _ClassTable(cls)
+        return cls
+
+    def _block_size(self, p):
+        nv, nx, ny = p.ic.shape
+        if p.b:
+            return p.b
+        if self.tuner:
+            return self.tuner.block_size(p.plugin, p.precision, p.its, nx, ny)
+        for b in (16, 32, 8):
+            if nx % b == 0 and ny % b == 0:
+                return b
+        raise ValueError(f"grid {nx}x{ny} has no supported block size")
+
+    def _admit(self):
+        now = self.now()
+        pending = sorted((t for t in self.queue if t.arrival <= now), key=lambda t: t.arrival)
+        for t in pending:
+            if self.max_active is not None and len(self.running) >= self.max_active:
+                break
+            p = t.problem
+            b = self._block_size(p)
+            cls = self._class_for(p, b)
+            nv, nx, ny = p.ic.shape
+            need = cls.ring_bytes(nx, ny) + 256
+            if self.in_use + need > self.hbm_budget:
+                if not self.running and need > self.hbm_budget:
+                    raise MemoryError(f"problem needs {need} B, more than the HBM budget {self.hbm_budget} B")
+                break
+            self.queue.remove(t)
+            self._start(t, cls, b, need)
+
+    def _start(self, t, cls, b, need):
+        p = t.problem
+        nv, nx, ny = p.ic.shape
+        if nv != cls.nv or nx % b or ny % b:
+            raise ValueError(f"problem shape {p.ic.shape} does not fit class {cls.key}")
+        N = schedule.requested_steps(p.globals[0], p.globals[1], p.globals[2])
+        t.times = schedule.swept_times(N, b, cls.ops, cls.its, exact=p.exact)
+        t.nx, t.ny = nx, ny
+        T = 2 * cls.P + cls.its
+        lvl = nv * nx * ny * cls.rb
+        t.ring = hip.DeviceArray(T * lvl)
+        ic = numpy.ascontiguousarray(p.ic, dtype=cls.dtype)
+        for A in range(cls.its):
+            t.ring.upload(ic, (A % T) * lvl)
+        hc = _HostConsts(cls.layout)
+        cls.cpu.set_globals(*p.globals, source_mod=hc)
+        t.consts = hip.DeviceArray(len(hc.buf))
+        hip.htod_bytes(t.consts.ptr, bytes(hc.buf))
+        G = t.times.G
+        t.c = 0 if G >= 0 else 1
+        t.c_end = G + 2 + t.times.tail
+        t.final_level = cls.its - 1 + t.times.stages + t.times.tail
+        t.cls = cls
+        t.nbytes = need
+        t.admitted = self.now()
+        self.in_use += need
+        self.peak_in_use = max(self.peak_in_use, self.in_use)
+        self.running.append(t)
+        tab = self.tables[cls.key]
+        tab.running.append(t)
+        tab.dirty = True
+        if t.c >= t.c_end:  # nothing to run
+            self._retire(t)
+
+    def _retire(self, t):
+        cls = t.cls
+        nv = cls.nv
+        T = 2 * cls.P + cls.its
+        lvl = nv * t.nx * t.ny * cls.rb
+        out = numpy.empty((nv, t.nx, t.ny), dtype=cls.dtype)
+        t.ring.download(out, (t.final_level % T) * lvl)
+        t.result = out.astype(numpy.float64)
+        t.finished = self.now()
+        t.ring.close()
+        t.consts.close()
+        t.ring = t.consts = None
+        self.in_use -= t.nbytes
+        self.running.remove(t)
+        tab = self.tables[cls.key]
+        tab.running.remove(t)
+        tab.dirty = True
+        self.finished.append(t)
+
+
+def percentile(xs, q):
+    xs = sorted(xs)
+    if not xs:
+        return None
+    k = (len(xs) - 1) * q / 100.0
+    f, c = math.floor(k), math.ceil(k)
+    return xs[f] if f == c else xs[f] + (xs[c] - xs[f]) * (k - f)

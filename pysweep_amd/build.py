@@ -32,9 +32,9 @@ def build_library():
 def precompile_plugins(verbose=False):
     """hiprtc-compile the kernel modules the tests, smoke() and bench.py use
     (no GPU needed) so the GPU host loads them from the cache."""
-    from pysweep_amd import kernels
+    from pysweep_amd import batch, kernels
     n = 0
-    for spec in kernels.shipped_module_specs():
+    for spec in kernels.shipped_module_specs() + batch.shipped_class_specs():
         kernels.build_module_code(spec)
         n += 1
         if verbose:

@@ -474,6 +474,55 @@ int ps_copy_f64(void *dst, const void *src, size_t n)
     return 0;
 }
 
+static int multi_round(const ps_multi_plan *p)
+{
+    void *insts = p->insts, *prefix = p->prefix, *sprefix = p->sprefix;
+    int n = p->ninst;
+    void *args[] = {&insts, &prefix, &n};
+    void *sargs[] = {&insts, &sprefix, &n};
+    int rc = 0;
+    if (p->total_tiles > 0) {
+        const unsigned g = static_cast<unsigned>(p->grid > 0 && p->grid < p->total_tiles ? p->grid : p->total_tiles);
+        const unsigned t = static_cast<unsigned>(p->threads);
+        // Within a round every problem keeps its own phase order:
+        // FirstPrism Up<Yb, UpPrism Xb<Oct<Yb, LastPrism Xb<Down.
+        if ((rc = launch_kind(p->up, g, t, args, PS_K_UP))) return rc;
+        if ((rc = launch_kind(p->xbridge, g, t, args, PS_K_XBRIDGE))) return rc;
+        if ((rc = launch_kind(p->octahedron, g, t, args, PS_K_OCTAHEDRON))) return rc;
+        if ((rc = launch_kind(p->down, g, t, args, PS_K_DOWN))) return rc;
+        if ((rc = launch_kind(p->ybridge, g, t, args, PS_K_YBRIDGE))) return rc;
+    }
+    if (p->total_stiles > 0 && p->standard) {
+        const unsigned g =
+            static_cast<unsigned>(p->sgrid > 0 && p->sgrid < p->total_stiles ? p->sgrid : p->total_stiles);
+        if ((rc = launch_kind(p->standard, g, static_cast<unsigned>(p->sthreads), sargs, PS_K_STANDARD))) return rc;
+    }
+    void *aargs[] = {&insts, &n};
+    return launch(p->advance, static_cast<unsigned>((n + 255) / 256), 256, aargs);
+}
+
+int ps_multi_run(const ps_multi_plan *plan, int rounds)
+{
+    PS_NEED_INIT();
+    if (!plan || !plan->insts || !plan->prefix || !plan->advance) return fail("incomplete multi plan");
+    if (plan->total_tiles > 0 && (!plan->up || !plan->ybridge || !plan->xbridge || !plan->octahedron || !plan->down))
+        return fail("multi plan without phase kernels");
+    for (int r = 0; r < rounds; ++r) {
+        int rc = multi_round(plan);
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int ps_multi_run_timed(const ps_multi_plan *plan, int rounds, ps_run_stats *stats)
+{
+    PS_NEED_INIT();
+    timer_begin(stats);
+    int rc = ps_multi_run(plan, rounds);
+    int rc2 = timer_end();
+    return rc ? rc : rc2;
+}
+
 int ps_graph_begin(void)
 {
     PS_NEED_INIT();

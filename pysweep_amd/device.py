@@ -15,9 +15,10 @@ from pysweep_amd import hip, kernels, plugins, schedule
 
 
 class _Run:
-    def __init__(self, nv, its, nx, ny, T):
+    def __init__(self, nv, its, nx, ny, T, real="double"):
         self.nv, self.its, self.nx, self.ny, self.T = nv, its, nx, ny, T
-        self.level_bytes = nv * nx * ny * 8
+        self.dtype = numpy.float64 if real == "double" else numpy.float32
+        self.level_bytes = nv * nx * ny * numpy.dtype(self.dtype).itemsize
         self.ring = hip.DeviceArray(T * self.level_bytes)
         self.modules = []
 
@@ -29,7 +30,7 @@ class _Run:
         cpu_module.set_globals(*globals_, source_mod=self.facade())
 
     def upload_ic(self, ic):
-        ic = numpy.ascontiguousarray(ic, dtype=numpy.float64)
+        ic = numpy.ascontiguousarray(ic, dtype=self.dtype)
         if ic.shape != (self.nv, self.nx, self.ny):
             raise ValueError(f"initial condition shape {ic.shape} != {(self.nv, self.nx, self.ny)}")
         for A in range(self.its):
@@ -37,19 +38,24 @@ class _Run:
 
     def keep_ic_on_device(self, ic):
         """Device copy of the initial condition for stream-ordered resets."""
-        ic = numpy.ascontiguousarray(ic, dtype=numpy.float64)
+        ic = numpy.ascontiguousarray(ic, dtype=self.dtype)
         self.ic_dev = hip.DeviceArray(ic.nbytes)
         self.ic_dev.upload(ic)
 
     def reset(self):
         """Levels 0..s-1 <- initial condition, on the work stream (no host copy)."""
-        n = self.level_bytes // 8
+        n = self.level_bytes // 8  # copied as 8-byte words
         for A in range(self.its):
             hip.copy_f64(self.ring.ptr + (A % self.T) * self.level_bytes, self.ic_dev.ptr, n)
 
     def level(self, A, out=None):
-        out = numpy.empty((self.nv, self.nx, self.ny)) if out is None else out
-        return self.ring.download(out, (A % self.T) * self.level_bytes)
+        """Level A as float64 (written into ``out`` when given)."""
+        buf = numpy.empty((self.nv, self.nx, self.ny), dtype=self.dtype)
+        self.ring.download(buf, (A % self.T) * self.level_bytes)
+        if out is None:
+            return buf.astype(numpy.float64, copy=False)
+        out[...] = buf
+        return out
 
     def close(self):
         self.ring.close()
@@ -58,9 +64,9 @@ class _Run:
 class StandardRun(_Run):
     """Standard decomposition: one launch of ``Standard`` per stage."""
 
-    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, T=None):
-        super().__init__(nv, its, nx, ny, T or its + 1)
-        self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads)
+    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, T=None, real="double"):
+        super().__init__(nv, its, nx, ny, T or its + 1, real)
+        self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads, real=real)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         self.plan = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=self.spec.bx,
@@ -86,15 +92,16 @@ class StandardRun(_Run):
 class SweptRun(_Run):
     """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
 
-    def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True):
+    def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True,
+                 real="double"):
         P = b // (2 * ops) - 1
         if b % (2 * ops) or P < 1:
             raise ValueError(f"invalid blocksize {b} for operating_points {ops}")
         if nx % b or ny % b:
             raise ValueError(f"grid {nx}x{ny} is not divisible by the blocksize {b}")
-        super().__init__(nv, its, nx, ny, 2 * P + its)
+        super().__init__(nv, its, nx, ny, 2 * P + its, real)
         self.P, self.b, self.ops = P, b, ops
-        self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads)
+        self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads, real=real)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         f = {n: mod.get_function(n) for n in kernels.SWEPT_KERNELS}
@@ -103,7 +110,7 @@ class SweptRun(_Run):
                                   xbridge=f["XBridge"], octahedron=f["Octahedron"], down=f["DownPyramid"])
         self.tail = None
         if tail_standard:
-            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny)
+            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, real=real)
             smod = kernels.load(sspec)
             self.modules.append(smod)
             self.tail = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=sspec.bx,

@@ -29,7 +29,9 @@ struct eu_q {
     double r, m, n, e; /* rho, rho*u, rho*v, E */
 };
 
-__device__ __forceinline__ eu_q eu_at(const double *s, int i)
+/* state values are read into double whatever the storage type R */
+template <class R>
+__device__ __forceinline__ eu_q eu_at(const R *s, int i)
 {
     eu_q q;
     q.r = s[i];
@@ -108,8 +110,8 @@ __device__ __forceinline__ eu_q eu_dissipation(const eu_q &L, const eu_q &R)
 
 /* flux difference along one direction (euler.py:158-179, 202-223): the
  * west face's flux and dissipation added, the east face's subtracted, *0.5 */
-template <bool XDIR>
-__device__ __forceinline__ eu_q eu_divergence(const double *s, int idx, int stride)
+template <bool XDIR, class S>
+__device__ __forceinline__ eu_q eu_divergence(const S *s, int idx, int stride)
 {
     const eu_q ww = eu_at(s, idx - 2 * stride), w = eu_at(s, idx - stride), c = eu_at(s, idx);
     const eu_q e = eu_at(s, idx + stride), ee = eu_at(s, idx + 2 * stride);
@@ -138,23 +140,30 @@ __device__ __forceinline__ eu_q eu_divergence(const double *s, int idx, int stri
 
 /* midpoint RK2 (euler.py:131-151): final stages (even globalTimeStep) start
  * from the level one back with the full step, intermediate stages from the
- * current level with half a step. */
-__device__ void step(double *shared_state, int idx, int globalTimeStep)
+ * current level with half a step.  Arithmetic in double; R is the storage
+ * type of the state. */
+template <class R>
+__device__ __forceinline__ void eu_step(R *shared_state, int idx, int globalTimeStep)
 {
     const eu_q fx = eu_divergence<true>(shared_state, idx, SY);
     const eu_q fy = eu_divergence<false>(shared_state, idx, 1);
-    double *out = shared_state + idx + TIMES;
+    R *out = shared_state + idx + TIMES;
+    const eu_q b = eu_at(shared_state, globalTimeStep % 2 == 0 ? idx - TIMES : idx);
+    const double h = globalTimeStep % 2 == 0 ? 1.0 : 0.5;
     if (globalTimeStep % 2 == 0) {
-        const double *b = shared_state + idx - TIMES;
-        out[0] = b[0] + (fx.r * DTDX + fy.r * DTDY);
-        out[VARS] = b[VARS] + (fx.m * DTDX + fy.m * DTDY);
-        out[2 * VARS] = b[2 * VARS] + (fx.n * DTDX + fy.n * DTDY);
-        out[3 * VARS] = b[3 * VARS] + (fx.e * DTDX + fy.e * DTDY);
+        out[0] = (R)(b.r + (fx.r * DTDX + fy.r * DTDY));
+        out[VARS] = (R)(b.m + (fx.m * DTDX + fy.m * DTDY));
+        out[2 * VARS] = (R)(b.n + (fx.n * DTDX + fy.n * DTDY));
+        out[3 * VARS] = (R)(b.e + (fx.e * DTDX + fy.e * DTDY));
     } else {
-        const double *b = shared_state + idx;
-        out[0] = b[0] + 0.5 * (fx.r * DTDX + fy.r * DTDY);
-        out[VARS] = b[VARS] + 0.5 * (fx.m * DTDX + fy.m * DTDY);
-        out[2 * VARS] = b[2 * VARS] + 0.5 * (fx.n * DTDX + fy.n * DTDY);
-        out[3 * VARS] = b[3 * VARS] + 0.5 * (fx.e * DTDX + fy.e * DTDY);
+        out[0] = (R)(b.r + h * (fx.r * DTDX + fy.r * DTDY));
+        out[VARS] = (R)(b.m + h * (fx.m * DTDX + fy.m * DTDY));
+        out[2 * VARS] = (R)(b.n + h * (fx.n * DTDX + fy.n * DTDY));
+        out[3 * VARS] = (R)(b.e + h * (fx.e * DTDX + fy.e * DTDY));
     }
+}
+
+__device__ void step(double *shared_state, int idx, int globalTimeStep)
+{
+    eu_step(shared_state, idx, globalTimeStep);
 }

@@ -19,22 +19,30 @@ __device__ __constant__ double ALPHA;
 __device__ __constant__ double COURANT;
 __device__ __constant__ int SCHEME; /* 1: forward Euler, 0: RK2 */
 
-__device__ double heat_laplace(const double *u, int idx)
+/* arithmetic in double; R is the storage type of the state */
+template <class R>
+__device__ __forceinline__ double heat_laplace(const R *u, int idx)
 {
     const double c = u[idx];
-    const double lx = COURANT * (u[idx + SY] - 2 * c + u[idx - SY]);
-    const double ly = COURANT * (u[idx + 1] - 2 * c + u[idx - 1]);
+    const double lx = COURANT * ((double)u[idx + SY] - 2 * c + (double)u[idx - SY]);
+    const double ly = COURANT * ((double)u[idx + 1] - 2 * c + (double)u[idx - 1]);
     return lx + ly;
+}
+
+template <class R>
+__device__ __forceinline__ void heat_step(R *shared_state, int idx, int globalTimeStep)
+{
+    const double d = heat_laplace(shared_state, idx);
+    if (SCHEME) {
+        shared_state[idx + TIMES] = (R)(d + (double)shared_state[idx]);
+    } else if (globalTimeStep % 2 == 0) {
+        shared_state[idx + TIMES] = (R)(1.0 * d + (double)shared_state[idx - TIMES]);
+    } else {
+        shared_state[idx + TIMES] = (R)(0.5 * d + (double)shared_state[idx]);
+    }
 }
 
 __device__ void step(double *shared_state, int idx, int globalTimeStep)
 {
-    const double d = heat_laplace(shared_state, idx);
-    if (SCHEME) {
-        shared_state[idx + TIMES] = d + shared_state[idx];
-    } else if (globalTimeStep % 2 == 0) {
-        shared_state[idx + TIMES] = 1.0 * d + shared_state[idx - TIMES];
-    } else {
-        shared_state[idx + TIMES] = 0.5 * d + shared_state[idx];
-    }
+    heat_step(shared_state, idx, globalTimeStep);
 }

@@ -45,6 +45,21 @@ class StandardPlan(ctypes.Structure):
                 ("bx", c_int), ("by", c_int), ("threads", c_int), ("standard", c_void_p)]
 
 
+class Inst(ctypes.Structure):
+    """ps_inst: one problem of an instance-table launch."""
+    _fields_ = [("ring", c_void_p), ("consts", c_void_p), ("nx", c_int), ("ny", c_int), ("T", c_int),
+                ("tiles_x", c_int), ("tiles_y", c_int), ("stiles_x", c_int), ("stiles_y", c_int),
+                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int), ("pad_", c_int)]
+
+
+class MultiPlan(ctypes.Structure):
+    _fields_ = [("insts", c_void_p), ("prefix", c_void_p), ("sprefix", c_void_p), ("ninst", c_int),
+                ("total_tiles", c_int), ("total_stiles", c_int), ("grid", c_int), ("sgrid", c_int),
+                ("threads", c_int), ("sthreads", c_int),
+                ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p), ("octahedron", c_void_p),
+                ("down", c_void_p), ("standard", c_void_p), ("advance", c_void_p)]
+
+
 class RunStats(ctypes.Structure):
     """Per-kernel launch time totals (index: KERNEL_KINDS)."""
     _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_longlong * 6)]
@@ -100,6 +115,8 @@ _swept_run_timed = _sig("ps_swept_run_timed", ctypes.POINTER(SweptPlan), c_int, 
 _standard_run_timed = _sig("ps_standard_run_timed", ctypes.POINTER(StandardPlan), c_int, c_int,
                            ctypes.POINTER(RunStats))
 _copy_f64 = _sig("ps_copy_f64", c_void_p, c_void_p, c_size_t)
+_multi_run = _sig("ps_multi_run", ctypes.POINTER(MultiPlan), c_int)
+_multi_run_timed = _sig("ps_multi_run_timed", ctypes.POINTER(MultiPlan), c_int, ctypes.POINTER(RunStats))
 _graph_begin = _sig("ps_graph_begin")
 _graph_end = _sig("ps_graph_end", ctypes.POINTER(c_void_p))
 _graph_launch = _sig("ps_graph_launch", c_void_p)
@@ -112,7 +129,7 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_launch", "ps_synchronize",
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_destroy",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
-           "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
+           "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
 
 
@@ -279,6 +296,13 @@ def swept_run_timed(plan, G, c_begin, c_end, stats):
 
 def standard_run_timed(plan, g_begin, g_end, stats):
     check(_standard_run_timed(ctypes.byref(plan), g_begin, g_end, ctypes.byref(stats)), "ps_standard_run_timed")
+
+
+def multi_run(plan, rounds, stats=None):
+    if stats is None:
+        check(_multi_run(ctypes.byref(plan), rounds), "ps_multi_run")
+    else:
+        check(_multi_run_timed(ctypes.byref(plan), rounds, ctypes.byref(stats)), "ps_multi_run_timed")
 
 
 def copy_f64(dst, src, n):

@@ -7,6 +7,7 @@ fetches from the cache) without a GPU; ``load`` needs one.
 """
 import dataclasses
 import os
+import re
 
 from pysweep_amd import compiler
 from pysweep_amd.equations import path as eq_path
@@ -15,6 +16,30 @@ from pysweep_amd.equations import path as eq_path
 #: for two workgroups (eight wavefronts) per CU.
 LDS_BUDGET = 64 * 1024
 LDS_MAX = 160 * 1024
+
+
+_INCLUDE_RE = re.compile(r'^[ \t]*#[ \t]*include[ \t]+"([^"]+)"[ \t]*$', re.M)
+
+
+def read_plugin_source(path, _seen=None):
+    """Plugin device source with ``#include "file"`` of sibling files
+    expanded in place (so the code-object cache key covers them)."""
+    path = os.path.abspath(path)
+    seen = set() if _seen is None else _seen
+    if path in seen:
+        return ""
+    seen.add(path)
+    with open(path) as f:
+        text = f.read()
+    here = os.path.dirname(path)
+
+    def sub(m):
+        inc = os.path.join(here, m.group(1))
+        if not os.path.exists(inc):
+            return m.group(0)
+        return f'#line 1 "{m.group(1)}"\n' + read_plugin_source(inc, seen) + f'\n#line 1 "{os.path.basename(path)}"\n'
+
+    return _INCLUDE_RE.sub(sub, text)
 
 
 @dataclasses.dataclass(frozen=True)
@@ -29,47 +54,79 @@ class ModuleSpec:
     bx: int = 0          # standard tile rows
     by: int = 0          # standard tile columns
     threads: int = 256
+    real: str = "double"  # state type in HBM and LDS ("double" or "float")
+    multi: bool = False   # instance-table kernels with per-instance constants
+    mpss: int = 1         # pyramid height (standard modules of a multi class)
+
+    @property
+    def real_bytes(self):
+        return 8 if self.real == "double" else 4
 
     def source(self):
-        with open(self.user_path) as f:
-            user = f.read()
+        user = read_plugin_source(self.user_path)
         name = os.path.basename(self.user_path)
         if self.kind == "swept":
-            return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads)
+            return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads,
+                                         real=self.real, multi=self.multi)
         if self.kind == "standard":
-            return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads)
+            return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
+                                            mpss=self.mpss, real=self.real, multi=self.multi)
         raise ValueError(self.kind)
+
+    def const_layout(self):
+        return compiler.constant_layout(read_plugin_source(self.user_path))
 
     def lds_bytes(self):
         if self.kind == "swept":
-            return self.slots * self.nv * self.b * self.b * 8
-        return (self.its + 1) * self.nv * (self.bx + 2 * self.ops) * (self.by + 2 * self.ops) * 8
+            return self.slots * self.nv * self.b * self.b * self.real_bytes
+        return (self.its + 1) * self.nv * (self.bx + 2 * self.ops) * (self.by + 2 * self.ops) * self.real_bytes
 
     def describe(self):
         g = (f"b={self.b} slots={self.slots}" if self.kind == "swept" else f"tile={self.bx}x{self.by}")
-        return f"{self.kind} {os.path.basename(self.user_path)} nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}"
+        return (f"{self.kind}{'-multi' if self.multi else ''} {os.path.basename(self.user_path)} {self.real} "
+                f"nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}")
 
 
-def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET):
+def default_slots(nv, its, b, P, real_bytes=8):
+    """LDS window for a swept tile.  Measured on MI355X (Euler 1024^2): with
+    a b=16 FP64 tile the full 8-level window (64 KB, two workgroups per CU)
+    ran 2.6 GCUPS and a 5-level sliding window (40 KB, four workgroups per
+    CU) 3.7 GCUPS; a b=32 tile ran best with 4 levels (128 KB).  So: the
+    largest window within ~40 KB, at least ITS+1 and ITS+2 levels."""
+    level = nv * b * b * real_bytes
+    full = 2 * P + its
+    slots = max(its + 1, min(full, (40 * 1024) // level))
+    if slots < its + 2 and (its + 2) * level <= LDS_MAX:
+        slots = min(full, its + 2)
+    return slots
+
+
+def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
+               multi=False):
     if b % (2 * ops) != 0:
         raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
     P = b // (2 * ops) - 1
     if P < 1:
         raise ValueError(f"blocksize {b} too small for operating_points={ops} (needs b >= {4 * ops})")
-    level = nv * b * b * 8
+    rb = 8 if real == "double" else 4
+    level = nv * b * b * rb
     full = 2 * P + its
     if slots is None:
-        slots = min(full, max(its + 1, lds_budget // level))
+        slots = default_slots(nv, its, b, P, rb)
+    slots = min(slots, full)
     if slots < its + 1:
         raise ValueError("an LDS window needs at least intermediate_steps+1 levels")
     if slots * level > LDS_MAX:
         raise ValueError(f"swept tile b={b} with {slots} levels needs {slots * level} B of LDS (> {LDS_MAX})")
     if threads is None:
         threads = max(64, min(256, (b * b + 63) // 64 * 64))
-    return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads)
+        if level > 16 * 1024:
+            threads = 512
+    return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads,
+                      real=real, multi=multi)
 
 
-def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET):
+def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8):
     """Tile (bx rows, by columns) for the standard kernel: most cells within
     the LDS budget, then rows of at least 32 doubles (coalesced wavefront
     loads), then the least halo re-reading."""
@@ -78,7 +135,7 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET):
         for bx in (32, 16, 8, 4, 2):
             if bx > nx or by > ny:
                 continue
-            lds = (its + 1) * nv * (bx + 2 * ops) * (by + 2 * ops) * 8
+            lds = (its + 1) * nv * (bx + 2 * ops) * (by + 2 * ops) * real_bytes
             if lds > lds_budget:
                 continue
             halo = (bx + 2 * ops) * (by + 2 * ops) / (bx * by)
@@ -90,12 +147,14 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET):
     return best[1], best[2]
 
 
-def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, lds_budget=LDS_BUDGET):
+def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, lds_budget=LDS_BUDGET,
+                  real="double", multi=False, mpss=1):
     if bx is None or by is None:
-        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget)
-    if bx + 2 * ops > nx + 2 * ops and bx > nx:
+        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4)
+    if bx > nx or by > ny:
         raise ValueError("standard tile larger than the grid")
-    return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads)
+    return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads,
+                      real=real, multi=multi, mpss=mpss)
 
 
 def build_module_code(spec):
@@ -137,6 +196,10 @@ def shipped_module_specs():
                              (ch, 1, 1, 1), (ch, 1, 2, 2), (hf, 1, 1, 2)):
         for n in (64, 384, 1024):
             specs.append(standard_spec(path, nv, k, s, n, n))
+    for name, nv, k, s in (("euler_fp32.cu", 4, 2, 2), ("heat_fp32.cu", 1, 1, 1)):
+        specs.append(standard_spec(eq_path(name), nv, k, s, 1024, 1024, real="float"))
+        for b in (16, 32):
+            specs.append(swept_spec(eq_path(name), nv, k, s, b, real="float"))
     seen, out = set(), []
     for s in specs:
         if s not in seen:

@@ -38,6 +38,18 @@ class GlobalHandle:
         return f"GlobalHandle({self.name!r}, {len(self.copies)} copies)"
 
 
+class HostField:
+    """A constant of one problem's host-side constant block (instance-table
+    modules); ``memcpy_htod`` writes into the block, which is uploaded when
+    the problem is admitted."""
+
+    def __init__(self, buf, offset, size, name):
+        self.buf, self.offset, self.size, self.name = buf, offset, size, name
+
+    def __repr__(self):
+        return f"HostField({self.name!r}, offset={self.offset}, size={self.size})"
+
+
 def _as_bytes(src):
     if isinstance(src, (bytes, bytearray, memoryview)):
         return bytes(src)
@@ -47,19 +59,28 @@ def _as_bytes(src):
     return numpy.ascontiguousarray(arr).tobytes()
 
 
+def _hip():
+    from pysweep_amd import hip
+    return hip
+
+
 class _Driver:
     """The subset of ``pycuda.driver`` the reference's plugins call."""
 
     def memcpy_htod(self, dst, src):
-        from pysweep_amd import hip
         data = _as_bytes(src)
+        if isinstance(dst, HostField):
+            if len(data) > dst.size:
+                raise ValueError(f"{len(data)} bytes do not fit device constant {dst.name!r} ({dst.size} bytes)")
+            dst.buf[dst.offset:dst.offset + len(data)] = data
+            return
         if isinstance(dst, GlobalHandle):
             for ptr, n in dst.copies:
                 if len(data) > n:
                     raise ValueError(f"{len(data)} bytes do not fit device global {dst.name!r} ({n} bytes)")
-                hip.htod_bytes(ptr, data)
+                _hip().htod_bytes(ptr, data)
         else:
-            hip.htod_bytes(int(dst), data)
+            _hip().htod_bytes(int(dst), data)
 
     def memcpy_dtoh(self, dst, src):
         from pysweep_amd import hip

@@ -1,0 +1,120 @@
+"""GPU tests of instance-table launches (batching, continuous admission,
+graph replay, FP32 state): every problem's result must not depend on what
+else runs; FP64 results equal the CPU oracle bitwise."""
+import numpy as np
+import pytest
+
+from gpu_helpers import normwise, o_euler, o_heat, oracle_stages
+
+pytestmark = pytest.mark.gpu
+
+
+def _euler_problem(nx, ny, steps, eps=3.0, xc=0.5, yc=-1.0, precision="fp64", b=None):
+    from pysweep_amd.batch import Problem
+    dx = 10.0 / nx
+    dt = 0.1 * dx
+    X, Y = np.meshgrid(np.linspace(-5, 5, nx, endpoint=False), np.linspace(-5, 5, ny, endpoint=False),
+                       indexing="ij")
+    ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=eps, xc=xc, yc=yc)
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    step = lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx)
+    return Problem("euler", ic, g, 2, precision=precision, b=b), step
+
+
+def _heat_problem(nx, ny, steps, fe=True, kx=1, ky=2, amp=1.3, precision="fp64", b=None):
+    from pysweep_amd.batch import Problem
+    dx = 1.0 / nx
+    dt = 0.1 * dx * dx
+    x = np.linspace(0, 1, nx, endpoint=False)
+    y = np.linspace(0, 1, ny, endpoint=False)
+    ic = (amp * np.outer(np.sin(2 * np.pi * kx * x), np.sin(2 * np.pi * ky * y)))[None]
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, fe]
+    c = o_heat.courant_py(1.0, dt, dx)
+    step = lambda q, qb, gg: o_heat.step_py(q[0], None if qb is None else qb[0], gg, c, fe)[None]
+    return Problem("heat", ic, g, 1 if fe else 2, precision=precision, b=b), step
+
+
+def _mix():
+    return [_euler_problem(64, 96, 9), _euler_problem(96, 64, 5, eps=4.0), _euler_problem(128, 128, 7),
+            _heat_problem(64, 64, 30), _heat_problem(128, 64, 17, kx=2), _heat_problem(64, 128, 12, fe=False)]
+
+
+def _run(problems, **kw):
+    from pysweep_amd.batch import Engine
+    eng = Engine(**kw)
+    tickets = [eng.submit(p, arrival=0.0) for p in problems]
+    eng.run()
+    return tickets, eng
+
+
+def test_batched_mix_matches_oracle_bitwise(device):
+    mix = _mix()
+    tickets, eng = _run([p for p, _ in mix])
+    for t, (p, step) in zip(tickets, mix):
+        steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
+        want = oracle_stages(step, p.ic, p.its, p.its * steps)
+        assert np.array_equal(t.result, want), (p.plugin, p.ic.shape)
+
+
+def test_alone_equals_in_mix(device):
+    mix = _mix()
+    tickets, _ = _run([p for p, _ in mix])
+    for i in (0, 4):
+        alone, _ = _run([mix[i][0]])
+        assert np.array_equal(alone[0].result, tickets[i].result)
+
+
+def test_graph_replay_equals_eager(device):
+    mix = _mix()
+    a, _ = _run([p for p, _ in mix], use_graphs=True)
+    b, _ = _run([p for p, _ in mix], use_graphs=False)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.result, y.result)
+
+
+def test_continuous_admission_under_hbm_budget(device):
+    """Staggered arrivals and a budget that fits about two problems: problems
+    queue, are admitted as memory frees up, and still match the oracle."""
+    from pysweep_amd.batch import Engine
+    mix = _mix()
+    eng = Engine(hbm_budget=3 * 8 * 4 * 128 * 128 * 8)
+    tickets = [eng.submit(p, arrival=0.002 * i) for i, (p, _) in enumerate(mix)]
+    eng.run()
+    assert eng.peak_in_use <= eng.hbm_budget
+    assert all(t.done for t in tickets)
+    for t, (p, step) in zip(tickets, mix):
+        steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
+        assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps))
+
+
+def test_block_size_choice_does_not_change_results(device):
+    (p16, step), (p32, _) = _euler_problem(128, 96, 11, b=16), _euler_problem(128, 96, 11, b=32)
+    (p8, _) = _euler_problem(128, 96, 11, b=8)
+    r, _ = _run([p16, p32, p8])
+    assert np.array_equal(r[0].result, r[1].result)
+    assert np.array_equal(r[0].result, r[2].result)
+
+
+def test_tail_only_problem(device):
+    """Fewer stages than one pyramid: every stage runs on the standard kernel."""
+    p, step = _euler_problem(64, 64, 1, b=32)
+    r, _ = _run([p])
+    assert np.array_equal(r[0].result, oracle_stages(step, p.ic, 2, 2))
+
+
+def test_fp32_state_error_against_fp64_oracle(device):
+    """FP32 state, FP64 arithmetic: L2 and Linf errors against the FP64
+    oracle, stated per result; heat within 1e-6 relative."""
+    (h32, hstep) = _heat_problem(128, 128, 40, precision="fp32")
+    (e32, estep) = _euler_problem(128, 128, 20, precision="fp32")
+    r, _ = _run([h32, e32])
+    want_h = oracle_stages(hstep, h32.ic, 1, 40)
+    want_e = oracle_stages(estep, e32.ic, 2, 40)
+    lin_h = normwise(r[0].result, want_h)
+    l2_h = float(np.linalg.norm(r[0].result - want_h) / np.linalg.norm(want_h))
+    lin_e = normwise(r[1].result, want_e)
+    l2_e = float(np.linalg.norm(r[1].result - want_e) / np.linalg.norm(want_e))
+    print(f"fp32 heat: L2 {l2_h:.2e} Linf {lin_h:.2e}; fp32 euler: L2 {l2_e:.2e} Linf {lin_e:.2e}")
+    assert lin_h <= 1e-6 and l2_h <= 1e-6
+    assert np.isfinite(r[1].result).all()
+    assert l2_e <= 1e-3

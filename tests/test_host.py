@@ -1,0 +1,209 @@
+"""CPU tests of the host side: the C-ABI library exports, run-time
+compilation (no GPU needed), plugin loading and the device facade, the YAML
+input, the schedule arithmetic and the per-instance constant blocks."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import swept as o_swept
+from pysweep_amd import compiler, kernels, plugins, schedule
+from pysweep_amd.equations import path as eq_path
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = "/root/reference/pysweep/equations"
+
+
+def _header_symbols():
+    with open(os.path.join(ROOT, "include", "pysweep_hip.h")) as f:
+        text = f.read()
+    return sorted(set(re.findall(r"\b(ps_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    from pysweep_amd import hip
+    declared = _header_symbols()
+    assert declared, "no declarations found"
+    lib = ctypes.CDLL(hip.LIBPATH)
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert not missing, missing
+    assert sorted(hip.EXPORTS) == declared
+
+
+def test_device_count_without_gpu_is_not_an_error():
+    from pysweep_amd import hip
+    assert hip.device_count() >= 0
+
+
+@pytest.mark.parametrize("name,nv,k,s", [("heat.cu", 1, 1, 1), ("euler.cu", 4, 2, 2), ("euler_fp32.cu", 4, 2, 2)])
+def test_shipped_plugins_compile_for_gfx950(name, nv, k, s):
+    real = "float" if "fp32" in name else "double"
+    for spec in (kernels.swept_spec(eq_path(name), nv, k, s, 16, real=real),
+                 kernels.standard_spec(eq_path(name), nv, k, s, 64, 64, real=real),
+                 kernels.swept_spec(eq_path(name), nv, k, s, 16, real=real, multi=True)):
+        code = kernels.build_module_code(spec)
+        assert code[:4] == b"\x7fELF"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
+@pytest.mark.parametrize("name,nv,k,s", [("heat.cu", 1, 1, 1), ("euler.cu", 4, 2, 2), ("example.cu", 1, 1, 1),
+                                         ("checker.cu", 1, 1, 1), ("half.cu", 1, 1, 2)])
+def test_reference_plugin_sources_compile_unchanged(name, nv, k, s, tmp_path):
+    """The reference's device plugins build unchanged behind the harness
+    (compile only; hiprtc needs no GPU)."""
+    spec = kernels.swept_spec(os.path.join(REF, name), nv, k, s, 16)
+    code = compiler.compile_cached(spec.source(), name, cache_dir=str(tmp_path))
+    assert code[:4] == b"\x7fELF"
+
+
+def test_compile_error_is_reported():
+    from pysweep_amd import hip
+    with pytest.raises(hip.HipError, match="error"):
+        hip.compile_source("this is not C++;", "bad.hip", list(compiler.OPTIONS))
+
+
+def test_constant_layout_and_block_macros():
+    src = open(eq_path("heat.cu")).read()
+    lay = compiler.constant_layout(src)
+    names = [f.name for f in lay.fields]
+    assert names == ["DX", "DY", "DT", "ALPHA", "COURANT", "SCHEME"]
+    assert [f.offset for f in lay.fields] == [0, 8, 16, 24, 32, 40]
+    assert lay.size == 48
+    pre, body, _ = compiler.multi_prelude_and_source(src)
+    assert "__constant__" not in "".join(l for l in body.splitlines() if not l.strip().startswith("//"))
+    assert "#define COURANT (ps_pc.COURANT)" in pre
+
+
+def test_host_constant_block_facade():
+    from pysweep_amd.batch import _HostConsts
+    lay = compiler.constant_layout(open(eq_path("heat.cu")).read())
+    hc = _HostConsts(lay)
+    mod = plugins.load_cpu_module(eq_path("heat.py"))
+    mod.set_globals(0.0, 1.0, 0.01, 0.1, 0.2, 0.5, False, source_mod=hc)
+    vals = np.frombuffer(bytes(hc.buf[:40]), dtype=np.float64)
+    assert list(vals) == [0.1, 0.2, 0.01, 0.5, 0.5 * 0.01 / 0.1 / 0.1]
+    assert np.frombuffer(bytes(hc.buf[40:44]), dtype=np.intc)[0] == 0
+
+
+def test_plugin_loader_requires_contract(tmp_path):
+    bad = tmp_path / "bad.py"
+    bad.write_text("def set_globals(*a, source_mod=None):\n    pass\n")
+    with pytest.raises(AttributeError, match="step"):
+        plugins.load_cpu_module(str(bad))
+
+
+def test_plugin_loader_injects_driver(tmp_path):
+    p = tmp_path / "p.py"
+    p.write_text("try:\n    import pycuda.driver as cuda\nexcept Exception:\n    pass\n"
+                 "def set_globals(*a, source_mod=None):\n    pass\n"
+                 "def step(state, iidx, t, g):\n    pass\n")
+    mod = plugins.load_cpu_module(str(p))
+    assert mod.cuda is plugins.driver
+
+
+@pytest.mark.parametrize("N,b,k,s", [(100, 8, 1, 1), (500, 16, 2, 2), (1000, 32, 1, 1), (1000, 32, 2, 2),
+                                     (7, 16, 2, 2), (1, 32, 2, 2)])
+def test_schedule_matches_reference_arithmetic(N, b, k, s):
+    ref = o_swept.time_step_data(0.0, float(N), 1.0, b, k, s)
+    got = schedule.swept_times(N, b, k, s, exact=False) if ref["G"] >= 0 else None
+    if got is not None:
+        assert (got.P, got.G, got.stages, got.nt, got.subtraction) == \
+            (ref["P"], ref["G"], ref["stages"], ref["nt"], ref["subtraction"])
+    ex = schedule.swept_times(N, b, k, s, exact=True)
+    assert ex.stages + ex.tail == s * N and ex.stages % ex.P == 0 and ex.tail < ex.P + s * N * (ex.G < 0)
+
+
+def test_phase_sets_match_oracle():
+    for b, k in ((8, 1), (16, 2), (32, 2), (24, 1)):
+        P = b // (2 * k) - 1
+        for kind in schedule.PHASES:
+            L = 2 * P if kind == "Octahedron" else P
+            for r in range(1, L + 1):
+                assert schedule.phase_set(kind, r, b, k, P) == o_swept.phase_rect(kind, r, b, k, P)
+        per_cycle = sum(schedule.phase_cell_stages(kd, b, k, P) for kd in ("XBridge", "Octahedron", "YBridge"))
+        assert per_cycle == P * b * b
+
+
+def test_yaml_input(tmp_path):
+    from pysweep_amd import Solver
+    y = tmp_path / "in.yaml"
+    y.write_text(f"""
+swept: True
+blocksize: 16
+share: 1
+dtype: float64
+globals: [0, 0.5, 0.1, 0.1, 0.1, 1.4]
+intermediate_steps: 2
+operating_points: 2
+cpu: {eq_path('euler.py')}
+gpu: {eq_path('euler.cu')}
+""")
+    s = Solver(np.zeros((4, 32, 32)), str(y))
+    assert s.blocksize == (16, 16, 1) and s.intermediate == 2 and s.operating == 2
+    assert callable(s.cpu.step) and s.gpu == eq_path("euler.cu")
+    s.createTimeStepData()
+    assert s.timeSteps == 5 and s.maxPyramidSize == 3
+
+
+def test_yaml_rejects_bad_blocksize(tmp_path):
+    from pysweep_amd import Solver
+    y = tmp_path / "in.yaml"
+    y.write_text("blocksize: 10\noperating_points: 2\n")
+    with pytest.raises(AssertionError, match="blocksize"):
+        Solver(np.zeros((1, 8, 8)), str(y))
+
+
+def test_solver_rejects_split_share():
+    from pysweep_amd import Solver
+    s = Solver(np.zeros((1, 16, 16)), sendWarning=False)
+    s.share = 0.5
+    s.globals = [0, 1, 0.1, 0.1, 0.1, True]
+    s.setCPU(eq_path("example.py"))
+    s.setGPU(eq_path("example.cu"))
+    with pytest.raises(NotImplementedError, match="share"):
+        s()
+
+
+@pytest.mark.parametrize("name", ["heat", "euler", "example", "checker", "half"])
+def test_cpu_plugin_halves_follow_oracle(name):
+    """The plugins' Python step (the share=0 contract) on a periodic block."""
+    from oracle import euler as o_euler
+    from oracle import heat as o_heat
+    from oracle import plugins as o_pl
+    mod = plugins.load_cpu_module(eq_path(name + ".py"))
+    n = 16
+    rng = np.random.default_rng(5)
+    iidx = [(i, j) for i in range(n) for j in range(n)]
+    if name == "euler":
+        q = o_euler.initial_condition(n, eps=3.0)
+        st = np.stack([q, q, q])
+        mod.set_globals(0.0, 1.0, 0.05, 10 / n, 10 / n, 1.4)
+        mod.step(st, iidx, 1, 1)
+        want = o_euler.step_py(q, q, 1, 1.4, 0.05, 10 / n, 10 / n)
+    elif name == "heat":
+        u = rng.uniform(-1, 1, (3, 1, n, n))
+        mod.set_globals(0.0, 1.0, 1e-4, 1 / n, 1 / n, 1.0, True)
+        st = u.copy()
+        mod.step(st, iidx, 1, 1)
+        want = o_heat.step_py(u[1, 0], None, 1, o_heat.courant_py(1.0, 1e-4, 1 / n), True)[None]
+    elif name == "example":
+        u = rng.uniform(-1, 1, (3, 1, n, n))
+        mod.set_globals(0, 1, 0.1, 0.1, 0.1, False)
+        st = u.copy()
+        mod.step(st, iidx, 1, 2)
+        want = o_pl.example_step(u[1], u[0], 2, False)
+    elif name == "checker":
+        u = rng.uniform(-1, 1, (3, 1, n, n))
+        mod.set_globals(0, 1, 0.1, 0.1, 0.1, True)
+        st = u.copy()
+        mod.step(st, iidx, 1, 1)
+        want = o_pl.checker_step(u[1], u[0], 1)
+    else:
+        u = rng.uniform(-1, 1, (3, 1, n, n))
+        mod.set_globals(0, 1, 0.1, 0.1, 0.1)
+        st = u.copy()
+        mod.step(st, iidx, 1, 2)
+        want = o_pl.half_step(u[1], u[0], 2)
+    assert np.array_equal(st[2], want)
