@@ -46,10 +46,16 @@ def parse():
     ap.add_argument("--cpu-stages", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["swept", "standard", "both"], default="both")
+    ap.add_argument("--batch", type=int, default=512, help="problems of the batched workload (0: skip)")
+    ap.add_argument("--serve", type=int, default=48, help="requests of the served stream (0: skip)")
+    ap.add_argument("--rate", type=float, default=4.0, help="served-stream arrival rate per second")
+    ap.add_argument("--generic", action="store_true", help="per-cell plugin step instead of the level update")
+    ap.add_argument("--slots", type=int, default=None)
+    ap.add_argument("--threads", type=int, default=None)
     return ap.parse_args()
 
 
-def setup(n, b, nsteps):
+def setup(n, b, nsteps, generic=False, slots=None, threads=None):
     import numpy as np
 
     from oracle import euler as o_euler  # initial condition generator only
@@ -63,10 +69,10 @@ def setup(n, b, nsteps):
     g = [0.0, nsteps * dt, dt, dx, dx, 1.4]
     cpu = plugins.load_cpu_module(eq_path("euler.py"))
     cpu.set_globals(*g)
-    sw = device.SweptRun(eq_path("euler.cu"), 4, 2, 2, n, n, b)
+    sw = device.SweptRun(eq_path("euler.cu"), 4, 2, 2, n, n, b, generic=generic, slots=slots, threads=threads)
     sw.set_globals(cpu, g)
     sw.keep_ic_on_device(ic)
-    st = device.StandardRun(eq_path("euler.cu"), 4, 2, 2, n, n)
+    st = device.StandardRun(eq_path("euler.cu"), 4, 2, 2, n, n, generic=generic)
     st.set_globals(cpu, g)
     st.keep_ic_on_device(ic)
     times = schedule.swept_times(nsteps, b, 2, 2, exact=True)
@@ -107,13 +113,86 @@ def cpu_baseline(ic, g, stages):
             "seconds": sec, "np_version": np.__version__}
 
 
+def batched_workload(n, seed=0):
+    """Independent heat and Euler problems, 200 steps each, FP64: the batched
+    engine (instance-table launches, graph-replayed rounds) against solving
+    them one at a time with the plain standard kernel."""
+    from pysweep_amd import baseline, batch, hip, workloads
+    probs = workloads.batched_mix(n=n, steps=200, seed=seed)
+    cells = sum(p.ic.shape[1] * p.ic.shape[2] * 200 for p in probs)
+    out = {"problems": n, "steps": 200, "cell_steps": cells}
+    results = {}
+    for label, method in (("engine_swept", "swept"), ("engine_standard", "standard")):
+        for p in probs:
+            p.method = method
+        eng = batch.Engine()
+        for p in probs:
+            eng.submit(p, arrival=0.0)
+        hip.synchronize()
+        t0 = time.perf_counter()
+        eng.run()
+        hip.synchronize()
+        sec = time.perf_counter() - t0
+        results[label] = {t.problem.tag: t.result for t in eng.finished}
+        out[label] = {"seconds": sec, "solves_per_s": n / sec, "gcups": cells / sec / 1e9, "rounds": eng.rounds,
+                      "peak_hbm_gb": eng.peak_in_use / 1e9}
+    t0 = time.perf_counter()
+    same = 0
+    for p in probs:
+        r = baseline.solve_plain(p)
+        same += int(numpy_equal(r, results["engine_swept"][p.tag]) and numpy_equal(r, results["engine_standard"][p.tag]))
+    sec = time.perf_counter() - t0
+    out["one_at_a_time_plain"] = {"seconds": sec, "solves_per_s": n / sec, "gcups": cells / sec / 1e9}
+    out["bitwise_equal_results"] = same
+    best = max(out["engine_swept"]["solves_per_s"], out["engine_standard"]["solves_per_s"])
+    out["engine_over_plain"] = best / out["one_at_a_time_plain"]["solves_per_s"]
+    return out
+
+
+def numpy_equal(a, b):
+    import numpy as np
+    return bool(np.array_equal(a, b))
+
+
+def served_workload(n, rate, seed=1):
+    """Poisson arrivals of Euler vortex solves (sides 128..2048, 100..2000
+    steps, 30% FP32 state): the continuous-batching engine against serving
+    them one at a time, in arrival order, with the plain standard kernel."""
+    from pysweep_amd import baseline, batch, hip, workloads
+    stream = workloads.served_stream(n=n, rate=rate, seed=seed)
+    cells = sum(p.ic.shape[1] * p.ic.shape[2] * int(round((p.globals[1] - p.globals[0]) / p.globals[2]))
+                for _, p in stream)
+    out = {"requests": n, "rate_per_s": rate, "fp32_share": sum(p.precision == "fp32" for _, p in stream) / n,
+           "cell_steps": cells}
+    eng = batch.Engine(hbm_budget=96 << 30)
+    hip.synchronize()
+    eng.t0 = eng.clock()
+    tickets = [eng.submit(p, arrival=t) for t, p in stream]
+    eng.run()
+    hip.synchronize()
+    lat = [t.latency for t in tickets]
+    span = max(t.finished for t in tickets) - min(t.arrival for t in tickets)
+    out["engine"] = {"p50_s": batch.percentile(lat, 50), "p99_s": batch.percentile(lat, 99), "mean_s": sum(lat) / n,
+                     "solves_per_s": n / span, "gcups": cells / span / 1e9, "peak_hbm_gb": eng.peak_in_use / 1e9}
+    res = baseline.serve_one_at_a_time(stream)
+    lat = [f - a for a, f, _ in res]
+    span = max(f for _, f, _ in res) - min(a for a, _, _ in res)
+    out["one_at_a_time_plain"] = {"p50_s": batch.percentile(lat, 50), "p99_s": batch.percentile(lat, 99),
+                                  "mean_s": sum(lat) / n, "solves_per_s": n / span, "gcups": cells / span / 1e9}
+    by_tag = {t.problem.tag: t.result for t in tickets}
+    out["bitwise_equal_results"] = sum(numpy_equal(r, by_tag[p.tag])
+                                       for (_, p), (_, _, r) in zip(sorted(stream, key=lambda x: x[0]), res))
+    out["engine_over_plain_solves_per_s"] = out["engine"]["solves_per_s"] / out["one_at_a_time_plain"]["solves_per_s"]
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
               "(the x-strip multi-GPU decomposition is design only)", file=sys.stderr)
         sys.exit(2)
-    np, hip, ic, g, sw, st, times = setup(a.n, a.b, a.nsteps)
+    np, hip, ic, g, sw, st, times = setup(a.n, a.b, a.nsteps, a.generic, a.slots, a.threads)
     from pysweep_amd import schedule
     cells = a.n * a.n
     res = {}
@@ -133,7 +212,10 @@ def main():
     out["ms_per_step"] = res[head][0] * 1e3
     out["config"] = {"workload": f"euler-vortex-{a.n}x{a.n}-b{a.b}-{a.nsteps}steps-{head}", "nx": a.n, "ny": a.n,
                      "blocksize": a.b, "operating_points": 2, "intermediate_steps": 2, "steps_per_solve": a.nsteps,
-                     "method": head, "G": times.G, "swept_stages": times.stages, "tail_stages": times.tail}
+                     "method": head, "G": times.G, "swept_stages": times.stages, "tail_stages": times.tail,
+                     "lds_slots": sw.spec.slots, "threads": sw.spec.threads,
+                     "step": "per-cell plugin step" if a.generic else "plugin level update",
+                     "standard_tile": [st.spec.bx, st.spec.by]}
     if "standard" in gc:
         out["standard_gcups"] = gc["standard"]
         out["standard_ms_per_solve"] = res["standard"][0] * 1e3
@@ -170,6 +252,14 @@ def main():
         fin_sw = sw.level(times.stages + times.tail + 1)
         fin_st = st.level(2 * a.nsteps + 1)
         out["swept_equals_standard_bitwise"] = bool(np.array_equal(fin_sw, fin_st))
+    sw.close()
+    st.close()
+    if a.batch:
+        out["batched"] = batched_workload(a.batch)
+    if a.serve:
+        out["serving"] = served_workload(a.serve, a.rate)
+        out["p50_latency_s"] = out["serving"]["engine"]["p50_s"]
+        out["p99_latency_s"] = out["serving"]["engine"]["p99_s"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, g, a.cpu_stages)
     print(json.dumps(out), flush=True)

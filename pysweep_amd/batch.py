@@ -47,6 +47,7 @@ class Problem:
     b: int = None
     exact: bool = True
     tag: object = None
+    method: str = "swept"  # "swept", or "standard": every stage on the standard kernel
 
 
 class _HostConsts:
@@ -73,7 +74,8 @@ def class_specs(plugin, precision, its, b, slots=None, threads=None):
     cu = eq_path(cu64 if precision == "fp64" else cu32)
     P = b // (2 * ops) - 1
     spec = kernels.swept_spec(cu, nv, ops, its, b, slots=slots, threads=threads, real=real, multi=True)
-    bx, by = kernels.standard_tile(nv, ops, its, b * 4, b * 4, real_bytes=rb)
+    bx, by = kernels.standard_tile(nv, ops, its, b * 4, b * 4, real_bytes=rb,
+                                   level_update=kernels.has_level_update(cu))
     sspec = kernels.standard_spec(cu, nv, ops, its, max(bx, b), max(by, b), bx=bx, by=by, real=real, multi=True,
                                   mpss=P)
     return spec, sspec
@@ -312,7 +314,10 @@ class Engine:
         if nv != cls.nv or nx % b or ny % b:
             raise ValueError(f"problem shape {p.ic.shape} does not fit class {cls.key}")
         N = schedule.requested_steps(p.globals[0], p.globals[1], p.globals[2])
-        t.times = schedule.swept_times(N, b, cls.ops, cls.its, exact=p.exact)
+        if p.method == "standard":
+            t.times = schedule.standard_times(N, b, cls.ops, cls.its)
+        else:
+            t.times = schedule.swept_times(N, b, cls.ops, cls.its, exact=p.exact)
         t.nx, t.ny = nx, ny
         T = 2 * cls.P + cls.its
         lvl = nv * nx * ny * cls.rb

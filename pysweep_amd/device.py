@@ -64,9 +64,11 @@ class _Run:
 class StandardRun(_Run):
     """Standard decomposition: one launch of ``Standard`` per stage."""
 
-    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, T=None, real="double"):
+    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, T=None, real="double",
+                 generic=False):
         super().__init__(nv, its, nx, ny, T or its + 1, real)
-        self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads, real=real)
+        self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads, real=real,
+                                          generic=generic)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         self.plan = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=self.spec.bx,
@@ -93,7 +95,7 @@ class SweptRun(_Run):
     """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True,
-                 real="double"):
+                 real="double", generic=False):
         P = b // (2 * ops) - 1
         if b % (2 * ops) or P < 1:
             raise ValueError(f"invalid blocksize {b} for operating_points {ops}")
@@ -101,7 +103,8 @@ class SweptRun(_Run):
             raise ValueError(f"grid {nx}x{ny} is not divisible by the blocksize {b}")
         super().__init__(nv, its, nx, ny, 2 * P + its, real)
         self.P, self.b, self.ops = P, b, ops
-        self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads, real=real)
+        self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads, real=real,
+                                       generic=generic)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         f = {n: mod.get_function(n) for n in kernels.SWEPT_KERNELS}
@@ -110,7 +113,7 @@ class SweptRun(_Run):
                                   xbridge=f["XBridge"], octahedron=f["Octahedron"], down=f["DownPyramid"])
         self.tail = None
         if tail_standard:
-            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, real=real)
+            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, real=real, generic=generic)
             smod = kernels.load(sspec)
             self.modules.append(smod)
             self.tail = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=sspec.bx,

@@ -167,3 +167,143 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
 {
     eu_step(shared_state, idx, globalTimeStep);
 }
+
+/*
+ * Optional cooperative level update used by the pysweep_amd kernels when a
+ * plugin defines PS_HAS_LEVEL: all lanes of a workgroup evaluate one level
+ * of a rectangle of cells S = [x0,x1) x [y0,y1) together.  Each cell's
+ * pressure and each face's limited states, flux and dissipation are
+ * computed once (the per-cell step evaluates every face from both sides and
+ * ten pressures per cell) and shared through LDS; the sums are formed in the
+ * per-cell step's order from the same helper functions, so the results are
+ * bitwise those of step().
+ *
+ *   s      LDS tile, level r-1 at s[base + v*VARS + x*SY + y]; level r is
+ *          written at base + TIMES, the base of a final stage read at
+ *          base - TIMES;
+ *   sc     scratch of PS_LEVEL_SCRATCH(SX * SY, PS_MAXFACES) doubles,
+ *          PS_MAXFACES being the most faces of one direction of any
+ *          rectangle the harness passes ((x1-x0+1)*(y1-y0) or
+ *          (x1-x0)*(y1-y0+1)); the face buffer is component-major so that
+ *          consecutive lanes touch consecutive LDS words;
+ *   MAXC   upper bound of cells per lane, ceil(|S| / PS_NT).
+ */
+#define PS_HAS_LEVEL 1
+#define PS_LEVEL_SCRATCH(CELLS, FACES) ((CELLS) + 8 * (FACES))
+
+template <class R, int MAXC>
+__device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0, int y1, int globalTimeStep,
+                                         double *sc)
+{
+    const int h = x1 - x0, w = y1 - y0;
+    constexpr int FS = PS_MAXFACES;  /* component stride of the face buffer */
+    double *P = sc;                  /* pressures, tile-indexed x*SY + y */
+    double *F = sc + SX * SY;        /* face f, component c at F[c*FS + f]: flux 0-3, dissipation 4-7 */
+
+    /* 1. pressures on the cross-shaped footprint */
+    {
+        const int bh = h + 4, bw = w + 4, n = bh * bw;
+        for (int t = threadIdx.x; t < n; t += PS_NT) {
+            const int i = t / bw, j = t - (t / bw) * bw;
+            const bool inx = i >= 2 && i < h + 2, iny = j >= 2 && j < w + 2;
+            if (!(inx || iny)) continue;
+            const int x = x0 - 2 + i, y = y0 - 2 + j;
+            P[x * SY + y] = eu_pressure(eu_at(s, base + x * SY + y));
+        }
+    }
+    __syncthreads();
+
+    /* 2. x faces: face i (0..h) lies west of cell x0+i */
+    {
+        const int n = (h + 1) * w;
+        for (int t = threadIdx.x; t < n; t += PS_NT) {
+            const int i = t / w, j = t - (t / w) * w;
+            const int x = x0 + i, y = y0 + j;
+            const int idx = base + x * SY + y;
+            const eu_q wq = eu_at(s, idx - SY), c = eu_at(s, idx);
+            const double pww = P[(x - 2) * SY + y], pw = P[(x - 1) * SY + y];
+            const double pc = P[x * SY + y], pe = P[(x + 1) * SY + y];
+            const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
+            const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
+            const eu_q f = eu_flux<true>(L, Rr), d = eu_dissipation<true>(L, Rr);
+            double *o = F + t;
+            o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
+            o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
+        }
+    }
+    __syncthreads();
+
+    /* 3. x divergence of my cells (cell (i, j): west face i, east face i+1) */
+    eu_q dfx[MAXC];
+#pragma unroll
+    for (int m = 0; m < MAXC; ++m) {
+        const int t = threadIdx.x + m * PS_NT;
+        if (t < h * w) {
+            const int i = t / w, j = t - (t / w) * w;
+            const double *fw = F + (i * w + j), *fe = F + ((i + 1) * w + j);
+            eu_q a;
+            a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
+            a.r = a.r + fw[0]; a.m = a.m + fw[FS]; a.n = a.n + fw[2 * FS]; a.e = a.e + fw[3 * FS];
+            a.r = a.r + fw[4 * FS]; a.m = a.m + fw[5 * FS]; a.n = a.n + fw[6 * FS]; a.e = a.e + fw[7 * FS];
+            a.r = a.r - fe[0]; a.m = a.m - fe[FS]; a.n = a.n - fe[2 * FS]; a.e = a.e - fe[3 * FS];
+            a.r = a.r - fe[4 * FS]; a.m = a.m - fe[5 * FS]; a.n = a.n - fe[6 * FS]; a.e = a.e - fe[7 * FS];
+            a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
+            dfx[m] = a;
+        }
+    }
+    __syncthreads();
+
+    /* 4. y faces: face j (0..w) lies south of cell y0+j */
+    {
+        const int wf = w + 1, n = h * wf;
+        for (int t = threadIdx.x; t < n; t += PS_NT) {
+            const int i = t / wf, j = t - (t / wf) * wf;
+            const int x = x0 + i, y = y0 + j;
+            const int idx = base + x * SY + y;
+            const eu_q wq = eu_at(s, idx - 1), c = eu_at(s, idx);
+            const double pww = P[x * SY + y - 2], pw = P[x * SY + y - 1];
+            const double pc = P[x * SY + y], pe = P[x * SY + y + 1];
+            const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
+            const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
+            const eu_q f = eu_flux<false>(L, Rr), d = eu_dissipation<false>(L, Rr);
+            double *o = F + t;
+            o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
+            o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
+        }
+    }
+    __syncthreads();
+
+    /* 5. y divergence and the RK2 update of my cells */
+    const bool fin = globalTimeStep % 2 == 0;
+    const double hh = fin ? 1.0 : 0.5;
+#pragma unroll
+    for (int m = 0; m < MAXC; ++m) {
+        const int t = threadIdx.x + m * PS_NT;
+        if (t < h * w) {
+            const int i = t / w, j = t - (t / w) * w;
+            const double *fs = F + (i * (w + 1) + j), *fn = fs + 1;
+            eu_q a;
+            a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
+            a.r = a.r + fs[0]; a.m = a.m + fs[FS]; a.n = a.n + fs[2 * FS]; a.e = a.e + fs[3 * FS];
+            a.r = a.r + fs[4 * FS]; a.m = a.m + fs[5 * FS]; a.n = a.n + fs[6 * FS]; a.e = a.e + fs[7 * FS];
+            a.r = a.r - fn[0]; a.m = a.m - fn[FS]; a.n = a.n - fn[2 * FS]; a.e = a.e - fn[3 * FS];
+            a.r = a.r - fn[4 * FS]; a.m = a.m - fn[5 * FS]; a.n = a.n - fn[6 * FS]; a.e = a.e - fn[7 * FS];
+            a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
+            const eu_q fx = dfx[m];
+            const int idx = base + (x0 + i) * SY + (y0 + j);
+            const eu_q b = eu_at(s, fin ? idx - TIMES : idx);
+            R *out = s + idx + TIMES;
+            if (fin) {
+                out[0] = (R)(b.r + (fx.r * DTDX + a.r * DTDY));
+                out[VARS] = (R)(b.m + (fx.m * DTDX + a.m * DTDY));
+                out[2 * VARS] = (R)(b.n + (fx.n * DTDX + a.n * DTDY));
+                out[3 * VARS] = (R)(b.e + (fx.e * DTDX + a.e * DTDY));
+            } else {
+                out[0] = (R)(b.r + hh * (fx.r * DTDX + a.r * DTDY));
+                out[VARS] = (R)(b.m + hh * (fx.m * DTDX + a.m * DTDY));
+                out[2 * VARS] = (R)(b.n + hh * (fx.n * DTDX + a.n * DTDY));
+                out[3 * VARS] = (R)(b.e + hh * (fx.e * DTDX + a.e * DTDY));
+            }
+        }
+    }
+}

@@ -57,6 +57,7 @@ class ModuleSpec:
     real: str = "double"  # state type in HBM and LDS ("double" or "float")
     multi: bool = False   # instance-table kernels with per-instance constants
     mpss: int = 1         # pyramid height (standard modules of a multi class)
+    generic: bool = False  # force the per-cell step even if the plugin has a level update
 
     @property
     def real_bytes(self):
@@ -67,10 +68,10 @@ class ModuleSpec:
         name = os.path.basename(self.user_path)
         if self.kind == "swept":
             return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads,
-                                         real=self.real, multi=self.multi)
+                                         real=self.real, multi=self.multi, generic=self.generic)
         if self.kind == "standard":
             return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
-                                            mpss=self.mpss, real=self.real, multi=self.multi)
+                                            mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic)
         raise ValueError(self.kind)
 
     def const_layout(self):
@@ -78,31 +79,51 @@ class ModuleSpec:
 
     def lds_bytes(self):
         if self.kind == "swept":
-            return self.slots * self.nv * self.b * self.b * self.real_bytes
-        return (self.its + 1) * self.nv * (self.bx + 2 * self.ops) * (self.by + 2 * self.ops) * self.real_bytes
+            tile = self.slots * self.nv * self.b * self.b * self.real_bytes
+            return tile + level_scratch_bytes(self.user_path, self.generic, self.b, self.b, swept_faces(self.b, self.ops))
+        sx, sy = self.bx + 2 * self.ops, self.by + 2 * self.ops
+        tile = (self.its + 1) * self.nv * sx * sy * self.real_bytes
+        return tile + level_scratch_bytes(self.user_path, self.generic, sx, sy, (self.bx + 1) * self.by)
 
     def describe(self):
         g = (f"b={self.b} slots={self.slots}" if self.kind == "swept" else f"tile={self.bx}x{self.by}")
-        return (f"{self.kind}{'-multi' if self.multi else ''} {os.path.basename(self.user_path)} {self.real} "
+        return (f"{self.kind}{'-multi' if self.multi else ''}{'-generic' if self.generic else ''} "
+                f"{os.path.basename(self.user_path)} {self.real} "
                 f"nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}")
 
 
-def default_slots(nv, its, b, P, real_bytes=8):
+def has_level_update(user_path):
+    """True when the plugin provides the cooperative level update."""
+    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_LEVEL\b", read_plugin_source(user_path), re.M) is not None
+
+
+def swept_faces(b, ops):
+    """Most faces of one direction over the swept sets of a b x b tile."""
+    m = b - 2 * ops
+    return (m + 1) * m
+
+
+def level_scratch_bytes(user_path, generic, sx, sy, faces):
+    """LDS of a plugin's level update: a tile of pressures and 8 doubles per face."""
+    if generic or not has_level_update(user_path):
+        return 0
+    return (sx * sy + 8 * faces) * 8
+
+
+def default_slots(nv, its, b, P, real_bytes=8, scratch=0):
     """LDS window for a swept tile.  Measured on MI355X (Euler 1024^2): with
-    a b=16 FP64 tile the full 8-level window (64 KB, two workgroups per CU)
-    ran 2.6 GCUPS and a 5-level sliding window (40 KB, four workgroups per
-    CU) 3.7 GCUPS; a b=32 tile ran best with 4 levels (128 KB).  So: the
-    largest window within ~40 KB, at least ITS+1 and ITS+2 levels."""
+    a b=16 FP64 tile and the per-cell step, the full 8-level window (64 KB,
+    two workgroups per CU) ran 2.6 GCUPS and a 5-level sliding window
+    (40 KB, four workgroups per CU) 3.7 GCUPS; with the plugin's level
+    update (12 KB scratch) 3 levels (36 KB) ran 4.5 GCUPS and 4 levels 3.9.
+    So: the largest window within ~40 KB of LDS, at least ITS+1 levels."""
     level = nv * b * b * real_bytes
     full = 2 * P + its
-    slots = max(its + 1, min(full, (40 * 1024) // level))
-    if slots < its + 2 and (its + 2) * level <= LDS_MAX:
-        slots = min(full, its + 2)
-    return slots
+    return max(its + 1, min(full, (40 * 1024 - scratch) // level))
 
 
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
-               multi=False):
+               multi=False, generic=False):
     if b % (2 * ops) != 0:
         raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
     P = b // (2 * ops) - 1
@@ -111,22 +132,24 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
     rb = 8 if real == "double" else 4
     level = nv * b * b * rb
     full = 2 * P + its
+    scratch = level_scratch_bytes(user_path, generic, b, b, swept_faces(b, ops))
     if slots is None:
-        slots = default_slots(nv, its, b, P, rb)
+        slots = default_slots(nv, its, b, P, rb, scratch)
     slots = min(slots, full)
     if slots < its + 1:
         raise ValueError("an LDS window needs at least intermediate_steps+1 levels")
-    if slots * level > LDS_MAX:
-        raise ValueError(f"swept tile b={b} with {slots} levels needs {slots * level} B of LDS (> {LDS_MAX})")
+    if slots * level + scratch > LDS_MAX:
+        raise ValueError(f"swept tile b={b} with {slots} levels needs {slots * level + scratch} B of LDS "
+                         f"(> {LDS_MAX})")
     if threads is None:
         threads = max(64, min(256, (b * b + 63) // 64 * 64))
         if level > 16 * 1024:
             threads = 512
     return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads,
-                      real=real, multi=multi)
+                      real=real, multi=multi, generic=generic)
 
 
-def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8):
+def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, level_update=False):
     """Tile (bx rows, by columns) for the standard kernel: most cells within
     the LDS budget, then rows of at least 32 doubles (coalesced wavefront
     loads), then the least halo re-reading."""
@@ -136,6 +159,8 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8):
             if bx > nx or by > ny:
                 continue
             lds = (its + 1) * nv * (bx + 2 * ops) * (by + 2 * ops) * real_bytes
+            if level_update:
+                lds += ((bx + 2 * ops) * (by + 2 * ops) + 8 * (bx + 1) * by) * 8
             if lds > lds_budget:
                 continue
             halo = (bx + 2 * ops) * (by + 2 * ops) / (bx * by)
@@ -148,13 +173,14 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8):
 
 
 def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, lds_budget=LDS_BUDGET,
-                  real="double", multi=False, mpss=1):
+                  real="double", multi=False, mpss=1, generic=False):
     if bx is None or by is None:
-        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4)
+        lu = not generic and has_level_update(user_path)
+        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, lu)
     if bx > nx or by > ny:
         raise ValueError("standard tile larger than the grid")
     return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads,
-                      real=real, multi=multi, mpss=mpss)
+                      real=real, multi=multi, mpss=mpss, generic=generic)
 
 
 def build_module_code(spec):

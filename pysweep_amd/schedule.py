@@ -59,6 +59,12 @@ def swept_times(N, b, k, s, exact=False):
     return SweptTimes(N=N, P=P, G=G, stages=swept, tail=tail, nt=nt, subtraction=sub, window=2 * P + s)
 
 
+def standard_times(N, b, k, s):
+    """Every one of the s*N stages on the standard kernel (no swept cycle)."""
+    P = b // (2 * k) - 1
+    return SweptTimes(N=N, P=P, G=-1, stages=0, tail=s * N, nt=N + 1, subtraction=0, window=2 * P + s)
+
+
 def complete_after_cycle(c, G, P, s):
     """Highest absolute level whose every cell is final after swept cycle c
     (0 = FirstPrism, 1..G = UpPrism, G+1 = LastPrism)."""

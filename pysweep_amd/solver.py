@@ -15,7 +15,9 @@ with the whole grid on the GPU, so ``share`` must be 1.
 Additions: ``exact`` (take exactly N steps instead of the reference's
 quantization to multiples of the pyramid height), ``history`` (keep every
 full-step level, the reference's output dataset ``data``), ``result`` (final
-full-step state), ``solve_seconds`` and ``gcups``.
+full-step state), ``solve_seconds`` and ``gcups``; ``tuning`` (launch shape:
+slots / threads / bx / by, and ``generic=True`` to call the plugin's per-cell
+step even when it provides a cooperative level update).
 """
 import os
 import time
@@ -201,10 +203,11 @@ class Solver(object):
         if self.simulation:
             self._run = device.SweptRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                         self.blocksize[0], slots=t.get("slots"), threads=t.get("threads"),
-                                        tail_standard=True)
+                                        tail_standard=True, generic=t.get("generic", False))
         else:
             self._run = device.StandardRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
-                                           bx=t.get("bx"), by=t.get("by"), threads=t.get("threads", 256))
+                                           bx=t.get("bx"), by=t.get("by"), threads=t.get("threads", 256),
+                                           generic=t.get("generic", False))
         # CPU globals, then the plugin's device constants (core/block.py:50, 90)
         self.cpu.set_globals(*self.globals)
         self._run.set_globals(self.cpu, self.globals)

@@ -40,6 +40,17 @@ def test_euler_matches_oracle(device, swept, b):
     assert np.array_equal(sol.final_state, want)
 
 
+@pytest.mark.parametrize("swept,b", [(False, 8), (True, 16), (True, 32)])
+def test_euler_per_cell_step_contract_matches_oracle(device, swept, b):
+    """The per-cell step(double*, int, int) path (what any plugin uses) gives
+    the same bits as the plugin's cooperative level update and the oracle."""
+    n, steps = 64, 12
+    ic, g, step = euler_case(n, steps, eps=5.0)
+    sol = gpu_solve("euler", ic, g, 2, 2, b, swept=swept, exact=True, tuning={"generic": True})
+    want = oracle_stages(step, ic, 2, 2 * steps)
+    assert np.array_equal(sol.final_state, want)
+
+
 def test_euler_one_stage_parity_with_reference_pow_along_trajectory(device):
     """Each GPU stage against one stage of the faithful numpy path (the C
     library's pow for squares, euler.py:194, 239) applied to the GPU's own
