@@ -108,6 +108,8 @@ typedef struct ps_grid {
     int xoff;     /* x lattice offset of this cycle (0 or b/2)            */
     int tiles_x;  /* tile rows                                            */
     int tiles_y;  /* tile columns                                         */
+    int keep_from; /* swept: levels >= keep_from are written in full; below
+                      it a phase skips the cells no later phase reads      */
 } ps_grid;
 
 /* One swept solve on a device-resident ring state[T][nv][nx][ny].
@@ -123,6 +125,10 @@ typedef struct ps_swept_plan {
     int its;        /* intermediate steps s                 */
     int T;          /* 2*P + s                              */
     int threads;    /* threads per workgroup                */
+    int store_all;  /* 1: write every computed cell to the ring (every level
+                       complete after its cycle, e.g. to record history);
+                       0: write only the cells a later phase reads and the
+                       final s levels                                      */
     ps_function up, ybridge, xbridge, octahedron, down;
 } ps_swept_plan;
 
@@ -140,7 +146,7 @@ typedef struct ps_standard_plan {
     void *state;
     int nx, ny;
     int its;
-    int T;          /* its + 1                              */
+    int T;          /* ring levels, at least its + 1        */
     int bx, by;     /* tile shape                           */
     int threads;
     ps_function standard;

@@ -1,0 +1,166 @@
+"""Command line: ``python -m pysweep_amd -f euler|heat -nx N -nt T -b B [--swept]``.
+
+Mirrors the reference's ``pysweep`` entry point (utils/commandline.py:28-144)
+on top of ``Solver``: the same options and problem set-up (Euler: vortex on
+[-5, 5]^2 sampled at ``linspace(-5, 5, nx)``, dx = 10/(nx-1), dt = 0.1 dx,
+gamma = 1.4, k = 2, s = 2; heat: the decaying mode on ``linspace(0, 1, nx)``,
+dx = 1/(nx-1), dt = 0.1 dx^2 / alpha, alpha = 1, k = 1, s = 1).  The
+reference writes the initial condition to an HDF5 file with h5py's default
+float32 dataset and reads it back; here it is built in memory and rounded to
+float32 the same way.  Each run appends a record to ``log.yaml`` (as
+core/io.py:10-22 does), with the device solve time and GCUPS added.
+
+Differences: ``--share`` must be 1 (the whole grid runs on the GPU), the
+figure options (-p) are not provided, and ``--exact`` takes exactly nt steps
+instead of the reference's quantization to whole swept cycles.
+"""
+import argparse
+import math
+import sys
+import warnings
+
+import numpy
+
+from pysweep_amd import equations
+
+
+def str2bool(v):
+    """utils/commandline.py:12-20."""
+    if isinstance(v, bool):
+        return v
+    if v.lower() in ("yes", "true", "t", "y", "1"):
+        return True
+    if v.lower() in ("no", "false", "f", "n", "0"):
+        return False
+    raise argparse.ArgumentTypeError("Boolean value expected as string.")
+
+
+def adjust_array_size(args):
+    """Round nx up to a multiple of the block size (commandline.py:22-26)."""
+    rem = args.spacesteps % args.blocksize
+    if rem:
+        args.spacesteps += args.blocksize - rem
+        warnings.warn(f"Adjusting array size to fit blocksize. New array size is {args.spacesteps}.")
+
+
+def euler_vortex(npx, npy, t=0.0, gamma=1.4, L=5.0):
+    """Isentropic vortex sampled on the reference's CLI grid, endpoints
+    included (equations/euler.py:33-56 with one rank; analytical
+    euler.py:63-105: eps = 5, M = sqrt(2/gamma), free stream (1, 1))."""
+    x = numpy.linspace(-L, L, npx, endpoint=True)
+    y = numpy.linspace(-L, L, npy, endpoint=True)
+    X, Y = numpy.meshgrid(x, y, indexing="ij")
+    minf = math.sqrt(2 / gamma)
+    beta = minf * 5.0 * math.sqrt(2) / 4 / math.pi * math.exp(0.5)
+    vinf = minf * math.sqrt(gamma)
+    u0, v0 = vinf * math.cos(math.pi / 4), vinf * math.sin(math.pi / 4)
+    xc, yc = u0 * t, v0 * t
+    dx0, dy0 = X - xc, Y - yc
+    om = beta * numpy.exp(-0.5 * (dx0 * dx0 + dy0 * dy0))
+    dT = -(gamma - 1) * om * om / 2
+    rho = (1 + dT) ** (1 / (gamma - 1))
+    u, v = u0 - dy0 * om, v0 + dx0 * om
+    p = 1 / gamma * (1 + dT) ** (gamma / (gamma - 1))
+    return numpy.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)])
+
+
+def heat_mode(npx, npy, t=0.0, alpha=1.0):
+    """sin(2 pi x) sin(2 pi y) exp(-8 pi^2 alpha t) on the reference's CLI
+    grid (equations/heat.py:66-93 with one rank)."""
+    x = numpy.linspace(0, 1, npx, endpoint=True)
+    y = numpy.linspace(0, 1, npy, endpoint=True)
+    u = numpy.outer(numpy.sin(2 * numpy.pi * x), numpy.sin(2 * numpy.pi * y)) * math.exp(-8 * math.pi ** 2 * alpha * t)
+    return u[None]
+
+
+def _solver(args, ic, globals_, ops, its, cpu, gpu, output):
+    from pysweep_amd.solver import Solver
+    solver = Solver(initialConditions=ic.astype(numpy.float32).astype(numpy.float64), sendWarning=not args.ignore)
+    solver.globals = globals_
+    solver.dtypeStr = "float64"
+    solver.dtype = numpy.dtype("float64")
+    solver.verbose = args.verbose
+    solver.operating = ops
+    solver.intermediate = its
+    solver.share = args.share
+    solver.simulation = bool(args.swept)
+    solver.blocksize = (args.blocksize, args.blocksize, 1)
+    solver.setCPU(equations.path(cpu))
+    solver.setGPU(equations.path(gpu))
+    solver.exid = []
+    solver.output = output
+    solver.exact = bool(args.exact)
+    solver.logfile = args.log
+    solver.loadCPUModule()
+    return solver
+
+
+def run_euler(args):
+    """utils/commandline.py:28-71."""
+    if args.ignore:
+        warnings.filterwarnings("ignore")
+    adjust_array_size(args)
+    gamma = 1.4
+    dx = 10 / (args.spacesteps - 1)
+    dt = 0.1 * dx
+    ic = euler_vortex(args.spacesteps, args.spacesteps, gamma=gamma)
+    solver = _solver(args, ic, [0, dt * args.timesteps, dt, dx, dx, gamma], 2, 2, "euler.py", "euler.cu",
+                     "eulerOutput.hdf5")
+    solver()
+    return solver
+
+
+def run_heat(args):
+    """utils/commandline.py:73-115."""
+    if args.ignore:
+        warnings.filterwarnings("ignore")
+    adjust_array_size(args)
+    alpha = 1
+    dx = 1 / (args.spacesteps - 1)
+    dt = float(0.1 * dx ** 2 / alpha)
+    ic = heat_mode(args.spacesteps, args.spacesteps, alpha=alpha)
+    solver = _solver(args, ic, [0, dt * args.timesteps, dt, dx, dx, alpha, True], 1, 1, "heat.py", "heat.cu",
+                     "heatOutput.hdf5")
+    solver()
+    return solver
+
+
+FUNCTIONS = {"euler": run_euler, "heat": run_heat}
+
+
+def parser():
+    p = argparse.ArgumentParser(prog="pysweep_amd")
+    p.add_argument("-f", "--function", nargs="?", default=None, choices=FUNCTIONS, type=str,
+                   help="which example to run")
+    p.add_argument("--swept", action="store_true", help="swept decomposition (default: standard)")
+    p.add_argument("-b", "--blocksize", default=12, nargs="?", type=int,
+                   help="block size b, a multiple of 2k (k points on each side of the operating point)")
+    p.add_argument("-nx", "--spacesteps", default=24, nargs="?", type=int, help="points in x and y")
+    p.add_argument("-nt", "--timesteps", default=10, nargs="?", type=int, help="time steps")
+    p.add_argument("-s", "--share", default=1.0, nargs="?", type=float, help="GPU share (must be 1)")
+    p.add_argument("--clean", action="store_true", help="accepted for compatibility; no files are written")
+    p.add_argument("--ignore", action="store_true", help="ignore warnings")
+    p.add_argument("-v", "--verbose", action="store_true")
+    p.add_argument("--exact", action="store_true", help="take exactly nt steps")
+    p.add_argument("--log", default="log.yaml", help="run log (YAML) to append to; '' for none")
+    return p
+
+
+def command_line(argv=None):
+    args = parser().parse_args(argv)
+    if args.function is None:
+        parser().print_usage()
+        return None
+    solver = FUNCTIONS[args.function](args)
+    print(f"{'swept' if solver.simulation else 'standard'} {args.function} {args.spacesteps}^2 "
+          f"b={args.blocksize}: {solver.stages} stages in {solver.solve_seconds:.4f} s on the GPU, "
+          f"{solver.gcups:.3f} GCUPS")
+    return solver
+
+
+def main():
+    command_line(sys.argv[1:])
+
+
+if __name__ == "__main__":
+    main()

@@ -3,6 +3,7 @@
 // include/pysweep_hip.h; every entry returns 0 or an error code and keeps the
 // message for ps_last_error().
 #include <hip/hip_runtime.h>
+#include <climits>
 #include <hip/hiprtc.h>
 
 #include <cstdio>
@@ -127,7 +128,7 @@ __global__ void ps_copy_kernel(double *__restrict__ dst, const double *__restric
 }
 
 // One phase launch: K(state, gts, A0, grid).
-int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xoff, int kind)
+int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xoff, int keep_from, int kind)
 {
     if (!f) return fail("swept plan has a null phase kernel");
     ps_grid g;
@@ -137,6 +138,7 @@ int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xof
     g.xoff = xoff;
     g.tiles_x = p->nx / p->b;
     g.tiles_y = p->ny / p->b;
+    g.keep_from = keep_from;
     void *state = p->state;
     void *args[] = {&state, &gts, &A0, &g};
     return launch_kind(f, static_cast<unsigned>(g.tiles_x * g.tiles_y), static_cast<unsigned>(p->threads), args,
@@ -382,26 +384,28 @@ static int swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
     if (p->T != 2 * P + s) return fail("ring must hold 2P+s levels");
     if (G < 0 || c_begin < 0 || c_end > G + 2 || c_begin > c_end) return fail("cycle range out of bounds");
     const int h = b / 2, base = s - 1;
+    // The final s levels (s-1+(G+1)P-s+1 ..) are always written in full.
+    const int keep = p->store_all ? INT_MIN : (G + 1) * P;
     for (int c = c_begin; c < c_end; ++c) {
         const int xoff = (c % 2) * h;
         int rc = 0;
         if (c == 0) {
             // FirstPrism: stages 1..P
-            if ((rc = launch_phase(p, p->up, 1, base, 0, PS_K_UP))) return rc;
-            if ((rc = launch_phase(p, p->ybridge, 1, base, 0, PS_K_YBRIDGE))) return rc;
+            if ((rc = launch_phase(p, p->up, 1, base, 0, keep, PS_K_UP))) return rc;
+            if ((rc = launch_phase(p, p->ybridge, 1, base, 0, keep, PS_K_YBRIDGE))) return rc;
         } else if (c <= G) {
             // UpPrism c: stages (c-1)P+1 .. (c+1)P
             const int gts = (c - 1) * P + 1;
             const int A = base + (c - 1) * P;
-            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff, PS_K_XBRIDGE))) return rc;
-            if ((rc = launch_phase(p, p->octahedron, gts, A, xoff, PS_K_OCTAHEDRON))) return rc;
-            if ((rc = launch_phase(p, p->ybridge, gts + P, A + P, xoff, PS_K_YBRIDGE))) return rc;
+            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff, keep, PS_K_XBRIDGE))) return rc;
+            if ((rc = launch_phase(p, p->octahedron, gts, A, xoff, keep, PS_K_OCTAHEDRON))) return rc;
+            if ((rc = launch_phase(p, p->ybridge, gts + P, A + P, xoff, keep, PS_K_YBRIDGE))) return rc;
         } else {
             // LastPrism: stages GP+1 .. (G+1)P
             const int gts = G * P + 1;
             const int A = base + G * P;
-            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff, PS_K_XBRIDGE))) return rc;
-            if ((rc = launch_phase(p, p->down, gts, A, xoff, PS_K_DOWN))) return rc;
+            if ((rc = launch_phase(p, p->xbridge, gts, A, xoff, keep, PS_K_XBRIDGE))) return rc;
+            if ((rc = launch_phase(p, p->down, gts, A, xoff, keep, PS_K_DOWN))) return rc;
         }
     }
     return 0;
@@ -434,6 +438,7 @@ static int standard_run(const ps_standard_plan *p, int gts_begin, int gts_end)
     g.xoff = 0;
     g.tiles_x = (p->nx + p->bx - 1) / p->bx;
     g.tiles_y = (p->ny + p->by - 1) / p->by;
+    g.keep_from = 0;
     void *state = p->state;
     for (int gts = gts_begin; gts < gts_end; ++gts) {
         int A = p->its - 2 + gts;

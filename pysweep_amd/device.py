@@ -126,6 +126,9 @@ class SweptRun(_Run):
         timings.  Returns the final absolute level."""
         G, P, s = times.G, self.P, self.its
         done = s - 1
+        # levels handed to on_level must be complete in the ring; otherwise
+        # the phases skip the cells that no later phase reads
+        self.plan.store_all = 1 if on_level is not None else 0
         if G >= 0:
             if stats is not None and on_level is None:
                 hip.swept_run_timed(self.plan, G, 0, G + 2, stats)

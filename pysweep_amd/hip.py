@@ -30,12 +30,12 @@ c_int, c_size_t, c_void_p, c_char_p = ctypes.c_int, ctypes.c_size_t, ctypes.c_vo
 
 class Grid(ctypes.Structure):
     _fields_ = [("nx", c_int), ("ny", c_int), ("T", c_int), ("xoff", c_int),
-                ("tiles_x", c_int), ("tiles_y", c_int)]
+                ("tiles_x", c_int), ("tiles_y", c_int), ("keep_from", c_int)]
 
 
 class SweptPlan(ctypes.Structure):
     _fields_ = [("state", c_void_p), ("nx", c_int), ("ny", c_int), ("b", c_int), ("ops", c_int),
-                ("its", c_int), ("T", c_int), ("threads", c_int),
+                ("its", c_int), ("T", c_int), ("threads", c_int), ("store_all", c_int),
                 ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p),
                 ("octahedron", c_void_p), ("down", c_void_p)]
 

@@ -105,6 +105,24 @@ def phase_set(kind, r, b, k, P):
     raise ValueError(kind)
 
 
+def private_set(kind, r, b, k, s, P):
+    """Cells of tile-local level r that no later phase reads from the ring
+    (the kernels' ``ps_private``): inside level r+1's set shrunk by k on each
+    side, and inside the sets of levels r+2..r+s.  None if empty."""
+    L = 2 * P if kind == "Octahedron" else P
+    if r + s > L:
+        return None
+    x0, x1, y0, y1 = phase_set(kind, r, b, k, P)
+    a = phase_set(kind, r + 1, b, k, P)
+    box = [max(x0, a[0] + k), min(x1, a[1] - k), max(y0, a[2] + k), min(y1, a[3] - k)]
+    for j in range(2, s + 1):
+        c = phase_set(kind, r + j, b, k, P)
+        box = [max(box[0], c[0]), min(box[1], c[1]), max(box[2], c[2]), min(box[3], c[3])]
+    if box[0] >= box[1] or box[2] >= box[3]:
+        return None
+    return tuple(box)
+
+
 def phase_cell_stages(kind, b, k, P):
     """Cell-stages one tile of a phase computes (sum of its set sizes)."""
     L = 2 * P if kind == "Octahedron" else P

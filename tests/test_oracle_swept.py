@@ -141,3 +141,65 @@ def test_every_cell_level_written_once():
     ring, fin = sw.swept_solve(lambda q, qb, g: plugins.example_step(q, qb, g, True),
                                np.zeros((1, n, n)), b, 1, 1, 4)
     assert np.all(ring.get(fin) == 3 * 5)
+
+
+def _hull(a, c):
+    if a is None:
+        return c
+    if c is None:
+        return a
+    return (min(a[0], c[0]), max(a[1], c[1]), min(a[2], c[2]), max(a[3], c[3]))
+
+
+@pytest.mark.parametrize("b,k,s", [(8, 1, 1), (8, 1, 2), (16, 2, 2), (16, 1, 2), (12, 1, 3), (24, 2, 3),
+                                   (32, 2, 2), (32, 1, 1), (24, 3, 2), (16, 2, 1), (32, 4, 2)])
+@pytest.mark.parametrize("G", [0, 1, 3])
+def test_private_cells_are_never_loaded(b, k, s, G):
+    """The swept kernels do not write the cells of a level that only the same
+    tile's next levels read (pysweep_amd.schedule.private_set, the kernels'
+    ps_private).  Replay every phase's HBM loads -- for each tile-local level
+    l the hull of the radius-k box around level l+1's set and of levels
+    l+2..l+s, minus level l's own set -- and check that no private cell below
+    the final s levels is ever loaded."""
+    from pysweep_amd import schedule as ps
+    P = b // (2 * k) - 1
+    n = 4 * b
+    names = {sw.UP: "UpPyramid", sw.YBRIDGE: "YBridge", sw.XBRIDGE: "XBridge", sw.OCT: "Octahedron",
+             sw.DOWN: "DownPyramid"}
+
+    def S(kind, r):
+        L = sw.phase_levels(kind, P)
+        return sw.phase_rect(kind, r, b, k, P) if 1 <= r <= L else None
+
+    def box(rect):
+        m = np.zeros((b, b), dtype=bool)
+        if rect is not None:
+            m[rect[0]:rect[1], rect[2]:rect[3]] = True
+        return m
+
+    last = s - 1 + (G + 1) * P
+    loaded = np.zeros((last + 1, n, n), dtype=bool)
+    phases = list(sw.schedule(b, k, s, G))
+    for kind, A0, _, xoff in phases:
+        yoff = sw.phase_yoff(kind, b)
+        for l in range(1 - s, sw.phase_levels(kind, P)):
+            u = S(kind, l + 1)
+            if u is not None:
+                u = (max(u[0] - k, 0), min(u[1] + k, b), max(u[2] - k, 0), min(u[3] + k, b))
+            for r in range(l + 2, l + s + 1):
+                u = _hull(u, S(kind, r))
+            m = box(u) & ~box(S(kind, l))
+            if A0 + l >= 0:
+                loaded[A0 + l] |= np.roll(np.tile(m, (n // b, n // b)), (xoff, yoff), axis=(0, 1))
+    skipped = 0
+    for kind, A0, _, xoff in phases:
+        yoff = sw.phase_yoff(kind, b)
+        for r in range(1, sw.phase_levels(kind, P) + 1):
+            q = ps.private_set(names[kind], r, b, k, s, P)
+            if q is None or A0 + r >= (G + 1) * P:
+                continue
+            m = sw.tile_mask(q, b, n, n, xoff, yoff)
+            assert not (m & loaded[A0 + r]).any(), (names[kind], r)
+            skipped += int(m.sum())
+    if G >= 1 and P >= 2:
+        assert skipped > 0
