@@ -180,7 +180,8 @@ class _ClassTable:
             st.tiles_x, st.tiles_y = t.nx // cls.b, t.ny // cls.b
             st.stiles_x, st.stiles_y = -(-t.nx // sbx), -(-t.ny // sby)
             st.G, st.tail, st.c, st.active = t.times.G, t.times.tail, t.c, 1
-            pre[i + 1] = pre[i] + st.tiles_x * st.tiles_y
+            # problems without swept cycles (G = -1) take no swept work items
+            pre[i + 1] = pre[i] + (st.tiles_x * st.tiles_y if t.times.G >= 0 else 0)
             spre[i + 1] = spre[i] + (st.stiles_x * st.stiles_y if t.times.tail else 0)
         hip.check(hip._htod(self.insts.ptr, ctypes.addressof(arr), ctypes.sizeof(arr)), "ps_memcpy_htod")
         self.prefix.upload(pre)

@@ -111,15 +111,15 @@ def level_scratch_bytes(user_path, generic, sx, sy, faces):
 
 
 def default_slots(nv, its, b, P, real_bytes=8, scratch=0):
-    """LDS window for a swept tile.  Measured on MI355X (Euler 1024^2): with
-    a b=16 FP64 tile and the per-cell step, the full 8-level window (64 KB,
-    two workgroups per CU) ran 2.6 GCUPS and a 5-level sliding window
-    (40 KB, four workgroups per CU) 3.7 GCUPS; with the plugin's level
-    update (12 KB scratch) 3 levels (36 KB) ran 4.5 GCUPS and 4 levels 3.9.
-    So: the largest window within ~40 KB of LDS, at least ITS+1 levels."""
+    """LDS window for a swept tile: the largest within ~28 KB including the
+    level scratch, at least ITS+1 levels.  Smaller windows slide more often
+    but let more tiles share a CU, which hides the HBM latency of each
+    level's face loads.  Measured on MI355X: Euler 1024^2 b=16 with the level
+    update (12 KB scratch) 3 levels 4.7 GCUPS, 4 levels 3.9; heat 8192^2
+    b=32 2 levels 175, 3 levels 180, 5 levels 163; b=64 2 levels 245."""
     level = nv * b * b * real_bytes
     full = 2 * P + its
-    return max(its + 1, min(full, (40 * 1024 - scratch) // level))
+    return max(its + 1, min(full, (28 * 1024 - scratch) // level))
 
 
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
@@ -149,10 +149,16 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
                       real=real, multi=multi, generic=generic)
 
 
-def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, level_update=False):
+def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, level_update=False, threads=256):
     """Tile (bx rows, by columns) for the standard kernel: most cells within
     the LDS budget, then rows of at least 32 doubles (coalesced wavefront
-    loads), then the least halo re-reading."""
+    loads), then the least halo re-reading.
+
+    With a plugin's cooperative level update every pass of the update (the
+    cells' pressures on the cross-shaped footprint, the x faces, the y faces)
+    must fit in one sweep of the workgroup's lanes: a second, mostly idle
+    sweep costs as much as the first.  Measured on MI355X (Euler 1024^2,
+    256 lanes): 8x16 7.1 GCUPS, 8x32 6.6, 16x16 6.5."""
     best = None
     for by in (64, 32, 16, 8):
         for bx in (32, 16, 8, 4, 2):
@@ -161,6 +167,9 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, lev
             lds = (its + 1) * nv * (bx + 2 * ops) * (by + 2 * ops) * real_bytes
             if level_update:
                 lds += ((bx + 2 * ops) * (by + 2 * ops) + 8 * (bx + 1) * by) * 8
+                foot = (bx + 2 * ops) * (by + 2 * ops) - 4 * ops * ops
+                if max((bx + 1) * by, bx * (by + 1), foot) > threads:
+                    continue
             if lds > lds_budget:
                 continue
             halo = (bx + 2 * ops) * (by + 2 * ops) / (bx * by)
@@ -176,7 +185,7 @@ def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256
                   real="double", multi=False, mpss=1, generic=False):
     if bx is None or by is None:
         lu = not generic and has_level_update(user_path)
-        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, lu)
+        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, lu, threads)
     if bx > nx or by > ny:
         raise ValueError("standard tile larger than the grid")
     return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads,

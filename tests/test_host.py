@@ -41,10 +41,30 @@ def test_device_count_without_gpu_is_not_an_error():
 def test_shipped_plugins_compile_for_gfx950(name, nv, k, s):
     real = "float" if "fp32" in name else "double"
     for spec in (kernels.swept_spec(eq_path(name), nv, k, s, 16, real=real),
+                 kernels.swept_spec(eq_path(name), nv, k, s, 32, real=real),
                  kernels.standard_spec(eq_path(name), nv, k, s, 64, 64, real=real),
                  kernels.swept_spec(eq_path(name), nv, k, s, 16, real=real, multi=True)):
         code = kernels.build_module_code(spec)
         assert code[:4] == b"\x7fELF"
+        for kernel, size in _private_segment_sizes(code).items():
+            assert size == 0, f"{spec.describe()}: {kernel} uses {size} B of scratch memory"
+
+
+READELF = "/opt/rocm/llvm/bin/llvm-readelf"
+
+
+def _private_segment_sizes(code):
+    """Scratch bytes per kernel from the code object's metadata note."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(READELF):
+        return {}
+    with tempfile.NamedTemporaryFile(suffix=".hsaco") as f:
+        f.write(code)
+        f.flush()
+        out = subprocess.run([READELF, "--notes", f.name], capture_output=True, text=True).stdout
+    return {m.group(1): int(m.group(2)) for m in
+            re.finditer(r"\.name:\s+(\w+).*?\.private_segment_fixed_size:\s+(\d+)", out, re.S)}
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference tree not present")
@@ -207,3 +227,41 @@ def test_cpu_plugin_halves_follow_oracle(name):
         mod.step(st, iidx, 1, 2)
         want = o_pl.half_step(u[1], u[0], 2)
     assert np.array_equal(st[2], want)
+
+
+def test_float_reciprocal_division_is_exact():
+    """The kernels' ps_div: q = int(float(t) * (1.0f / w)) corrected once is
+    t // w for 0 <= t < 2^22 (restated in float32 arithmetic)."""
+    rng = np.random.default_rng(3)
+    for w in list(range(1, 300)) + [1023, 1024, 1025, 4096, 65535]:
+        t = np.concatenate([np.arange(0, 4 * w + 3), rng.integers(0, 1 << 22, 2000)]).astype(np.int64)
+        rw = np.float32(1.0) / np.float32(w)
+        q = (t.astype(np.float32) * rw).astype(np.int64)
+        r = t - q * w
+        q = np.where(r < 0, q - 1, np.where(r >= w, q + 1, q))
+        assert np.array_equal(q, t // w), w
+
+
+def test_prefetch_register_bound_covers_every_face_load():
+    """PS_PF * threads >= the most face cells any swept phase loads after a
+    level (counted by brute force over the oracle's phase sets)."""
+    from oracle import swept as sw
+    from pysweep_amd import compiler
+    for b, k, s in [(8, 1, 1), (16, 2, 2), (32, 1, 1), (32, 2, 2), (16, 1, 2), (24, 3, 2), (12, 1, 3)]:
+        P = b // (2 * k) - 1
+        most = 0
+        for kind in (sw.UP, sw.YBRIDGE, sw.XBRIDGE, sw.OCT, sw.DOWN):
+            L = sw.phase_levels(kind, P)
+            for l in range(1, L):
+                need = np.zeros((b, b), dtype=bool)
+                x0, x1, y0, y1 = sw.phase_rect(kind, l + 1, b, k, P)
+                need[max(x0 - k, 0):x1 + k, max(y0 - k, 0):y1 + k] = True
+                for r in range(l + 2, min(l + s, L) + 1):
+                    x0, x1, y0, y1 = sw.phase_rect(kind, r, b, k, P)
+                    lo = np.argwhere(need).min(0)
+                    hi = np.argwhere(need).max(0) + 1
+                    need[min(lo[0], x0):max(hi[0], x1), min(lo[1], y0):max(hi[1], y1)] = True
+                x0, x1, y0, y1 = sw.phase_rect(kind, l, b, k, P)
+                need[x0:x1, y0:y1] = False
+                most = max(most, int(need.sum()))
+        assert compiler._face_cells(b, k, s) == most, (b, k, s)
