@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=512, help="problems of the batched workload (0: skip)")
     ap.add_argument("--serve", type=int, default=48, help="requests of the served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=4.0, help="served-stream arrival rate per second")
+    ap.add_argument("--large", type=int, default=1, help="large single instances (heat 8192^2, Euler 2048^2)")
     ap.add_argument("--generic", action="store_true", help="per-cell plugin step instead of the level update")
     ap.add_argument("--slots", type=int, default=None)
     ap.add_argument("--threads", type=int, default=None)
@@ -149,6 +150,95 @@ def batched_workload(n, seed=0):
     return out
 
 
+HEAT_FLOPS_PER_CELL_STAGE = 12      # heat.cu:11-24 as written
+
+
+def large_instances():
+    """Large single instances, 1000 steps, swept vs standard, FP64 and FP32
+    state: heat 8192^2 (b=32, and b=64) and Euler 2048^2 (b=32).  FP32
+    errors are stated against the FP64 result (itself bitwise equal to the
+    CPU oracle, which the parity tests check at smaller sizes)."""
+    import numpy as np
+
+    from pysweep_amd import device, hip, plugins, schedule, workloads
+    from pysweep_amd.equations import path as eq_path
+    out = []
+    cases = []
+    n = 8192
+    dx = 1.0 / n
+    dt = 0.1 * dx * dx
+    heat_g = [0.0, 1000 * dt, dt, dx, dx, 1.0, True]
+    cases.append(("heat", n, (32, 64), 1, 1, 1, heat_g, ("heat.cu", "heat_fp32.cu", "heat.py")))
+    n = 2048
+    dx = 10.0 / n
+    dt = 0.1 * dx
+    cases.append(("euler", n, (32,), 4, 2, 2, [0.0, 1000 * dt, dt, dx, dx, 1.4], ("euler.cu", "euler_fp32.cu",
+                                                                                 "euler.py")))
+    for name, n, bs, nv, k, s, g, (c64, c32, py) in cases:
+        ic = workloads.heat_mode(n, n) if name == "heat" else workloads.vortex_state(n, n)
+        cpu = plugins.load_cpu_module(eq_path(py))
+        cpu.set_globals(*g)
+        finals = {}
+        for real, cu in (("double", c64), ("float", c32)):
+            runs = [("standard", None)] + [("swept", b) for b in bs]
+            for method, b in runs:
+                if method == "swept":
+                    r = device.SweptRun(eq_path(cu), nv, k, s, n, n, b, real=real)
+                    times = schedule.swept_times(1000, b, k, s, exact=True)
+                    go = (lambda st, r=r, times=times: r.run(times, stats=st))
+                    last = times.stages + times.tail + s - 1
+                else:
+                    r = device.StandardRun(eq_path(cu), nv, k, s, n, n, real=real)
+                    go = (lambda st, r=r: r.run(1, s * 1000 + 1, stats=st))
+                    last = s * 1000 + s - 1
+                r.set_globals(cpu, g)
+                r.keep_ic_on_device(ic)
+                r.reset()
+                go(None)
+                r.reset()
+                hip.synchronize()
+                st = hip.RunStats()
+                t0 = time.perf_counter()
+                go(st)
+                hip.synchronize()
+                sec = time.perf_counter() - t0
+                finals[(real, method, b)] = r.level(last)
+                d = st.as_dict()
+                kern = max(d, key=lambda kk: d[kk]["ms"])
+                rb = 8 if real == "double" else 4
+                if kern == "Standard":
+                    cs = n * n
+                else:
+                    cs = (n // b) ** 2 * schedule.phase_cell_stages(kern, b, k, b // (2 * k) - 1)
+                flops = FLOPS_PER_CELL_STAGE if name == "euler" else HEAT_FLOPS_PER_CELL_STAGE
+                avg = d[kern]["avg_ms"] / 1e3
+                rec = {"case": f"{name}-{n}x{n}", "method": method, "b": b, "state": "fp64" if rb == 8 else "fp32",
+                       "gcups": n * n * 1000 / sec / 1e9, "seconds": sec, "dominant_kernel": kern,
+                       "dominant_avg_launch_ms": d[kern]["avg_ms"],
+                       "dominant_fp64_tflops": cs * flops / avg / 1e12,
+                       "dominant_fp64_frac": cs * flops / avg / 1e12 / FP64_PEAK_TFLOPS,
+                       # compulsory HBM bytes per step: heat 16 (read + write), Euler 160
+                       # (stage 1: 32 R + 32 W; stage 2: 64 R + 32 W), halved for FP32 state
+                       "algorithmic_gbs": n * n * 1000 * (16 if name == "heat" else 160) * rb / 8 / sec / 1e9}
+                rec["algorithmic_hbm_frac"] = rec["algorithmic_gbs"] / HBM_PEAK_GBS
+                out.append(rec)
+                r.close()
+        ref = finals[("double", "standard", None)]
+        for key, f in finals.items():
+            if key[0] == "double":
+                if not np.array_equal(f, ref):
+                    out.append({"case": f"{name}-{n}x{n}", "mismatch": list(map(str, key))})
+                continue
+            e = f - ref
+            for rec in out:
+                if rec.get("case") == f"{name}-{n}x{n}" and rec.get("state") == "fp32" and \
+                        (rec["method"], rec["b"]) == key[1:]:
+                    rec["fp32_rel_L2"] = float(np.sqrt((e * e).sum() / (ref * ref).sum()))
+                    rec["fp32_rel_Linf"] = float(np.abs(e).max() / np.abs(ref).max())
+        del finals
+    return out
+
+
 def numpy_equal(a, b):
     import numpy as np
     return bool(np.array_equal(a, b))
@@ -160,6 +250,8 @@ def served_workload(n, rate, seed=1):
     them one at a time, in arrival order, with the plain standard kernel."""
     from pysweep_amd import baseline, batch, hip, workloads
     stream = workloads.served_stream(n=n, rate=rate, seed=seed)
+    for _, p in stream:
+        p.method = "auto"  # the faster kernel per plugin class
     cells = sum(p.ic.shape[1] * p.ic.shape[2] * int(round((p.globals[1] - p.globals[0]) / p.globals[2]))
                 for _, p in stream)
     out = {"requests": n, "rate_per_s": rate, "fp32_share": sum(p.precision == "fp32" for _, p in stream) / n,
@@ -179,6 +271,21 @@ def served_workload(n, rate, seed=1):
     span = max(f for _, f, _ in res) - min(a for a, _, _ in res)
     out["one_at_a_time_plain"] = {"p50_s": batch.percentile(lat, 50), "p99_s": batch.percentile(lat, 99),
                                   "mean_s": sum(lat) / n, "solves_per_s": n / span, "gcups": cells / span / 1e9}
+    # saturated: every request queued at once (throughput, no arrival gaps)
+    eng = batch.Engine(hbm_budget=96 << 30)
+    hip.synchronize()
+    t0 = time.perf_counter()
+    for _, p in stream:
+        eng.submit(p, arrival=0.0)
+    eng.run()
+    hip.synchronize()
+    sat = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    baseline.serve_one_at_a_time([(0.0, p) for _, p in stream])
+    plain_sat = time.perf_counter() - t0
+    out["saturated"] = {"engine_solves_per_s": n / sat, "plain_solves_per_s": n / plain_sat,
+                        "engine_over_plain": plain_sat / sat, "engine_gcups": cells / sat / 1e9,
+                        "plain_gcups": cells / plain_sat / 1e9}
     by_tag = {t.problem.tag: t.result for t in tickets}
     out["bitwise_equal_results"] = sum(numpy_equal(r, by_tag[p.tag])
                                        for (_, p), (_, _, r) in zip(sorted(stream, key=lambda x: x[0]), res))
@@ -254,6 +361,8 @@ def main():
         out["swept_equals_standard_bitwise"] = bool(np.array_equal(fin_sw, fin_st))
     sw.close()
     st.close()
+    if a.large:
+        out["large_instances"] = large_instances()
     if a.batch:
         out["batched"] = batched_workload(a.batch)
     if a.serve:

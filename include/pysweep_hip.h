@@ -194,12 +194,18 @@ typedef struct ps_multi_plan {
     void *insts;        /* device ps_inst[ninst]                            */
     void *prefix;       /* device int[ninst+1]: prefix sums of swept tiles  */
     void *sprefix;      /* device int[ninst+1]: prefix sums of std tiles    */
+    void *owner;        /* device int[total_tiles]: problem of each tile    */
+    void *sowner;       /* device int[total_stiles]                         */
     int ninst;
     int total_tiles, total_stiles;
     int grid, sgrid;    /* persistent workgroups; 0 = one per tile          */
     int threads, sthreads;
     ps_function up, ybridge, xbridge, octahedron, down, standard, advance;
 } ps_multi_plan;
+
+/* Fill the owner tables of a plan from its prefix sums on the work stream
+ * (owner[t] = i for prefix[i] <= t < prefix[i+1]). */
+int ps_multi_index(const ps_multi_plan *plan);
 
 /* Run `rounds` rounds: each round launches UpPyramidMulti, XBridgeMulti,
  * OctahedronMulti, DownPyramidMulti, YBridgeMulti, StandardMulti and

@@ -56,6 +56,12 @@ class Tuner:
                 return cand
         raise ValueError(f"grid {nx}x{ny} has no supported block size")
 
+    def method(self, plugin, precision, its):
+        """"swept" or "standard" for problems submitted with method="auto"."""
+        from pysweep_amd import batch
+        r = self.records.get(_key(plugin, precision, its), {})
+        return r.get("method", batch.DEFAULT_METHOD.get((plugin, precision), "swept"))
+
     def config(self, plugin, precision, its, b):
         r = self.record(plugin, precision, its)
         out = {"grid_per_cu": r.get("grid_per_cu", 0)}

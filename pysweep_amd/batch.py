@@ -34,6 +34,19 @@ PLUGINS = {
 }
 
 
+#: method of a Problem with method="auto" when no tuner record says
+#: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
+#: side (Euler FP64 1024^2: standard 7.1 GCUPS, swept b=16 4.7; heat 8192^2
+#: b=32: standard 217, swept 180 -- swept wins only from b=64 on, which
+#: these grid sizes rarely allow).
+DEFAULT_METHOD = {
+    ("euler", "fp64"): "standard",
+    ("euler", "fp32"): "standard",
+    ("heat", "fp64"): "standard",
+    ("heat", "fp32"): "standard",
+}
+
+
 @dataclasses.dataclass(eq=False)
 class Problem:
     """One solve: plugin name, initial condition (nv, nx, ny), the plugin's
@@ -47,7 +60,7 @@ class Problem:
     b: int = None
     exact: bool = True
     tag: object = None
-    method: str = "swept"  # "swept", or "standard": every stage on the standard kernel
+    method: str = "swept"  # "swept"; "standard": every stage on the standard kernel; "auto": the faster one
 
 
 class _HostConsts:
@@ -134,6 +147,7 @@ class Ticket:
     admitted: float = None
     finished: float = None
     nbytes: int = 0
+    method: str = None
 
     @property
     def latency(self):
@@ -155,6 +169,7 @@ class _ClassTable:
         self.cap = 0
         self.plan = None
         self.graph = None
+        self.owner = self.sowner = None
 
     def rebuild(self, cus):
         cls = self.cls
@@ -186,14 +201,24 @@ class _ClassTable:
         hip.check(hip._htod(self.insts.ptr, ctypes.addressof(arr), ctypes.sizeof(arr)), "ps_memcpy_htod")
         self.prefix.upload(pre)
         self.sprefix.upload(spre)
+        # owner tables (problem of each work item), grown geometrically
+        for attr, total in (("owner", int(pre[-1])), ("sowner", int(spre[-1]))):
+            buf = getattr(self, attr, None)
+            if total and (buf is None or buf.nbytes < 4 * total):
+                if buf is not None:
+                    buf.close()
+                setattr(self, attr, hip.DeviceArray(8 * total))
         f = cls.fn
         grid = cls.grid_per_cu * cus if cls.grid_per_cu else 0
         self.plan = hip.MultiPlan(insts=self.insts.ptr, prefix=self.prefix.ptr, sprefix=self.sprefix.ptr,
+                                  owner=self.owner.ptr if getattr(self, "owner", None) else None,
+                                  sowner=self.sowner.ptr if getattr(self, "sowner", None) else None,
                                   ninst=n, total_tiles=int(pre[-1]), total_stiles=int(spre[-1]), grid=grid,
                                   sgrid=grid, threads=cls.spec.threads, sthreads=cls.sspec.threads,
                                   up=f["UpPyramid"], ybridge=f["YBridge"], xbridge=f["XBridge"],
                                   octahedron=f["Octahedron"], down=f["DownPyramid"], standard=f["Standard"],
                                   advance=f["advance"])
+        hip.multi_index(self.plan)
         self.graph = None
         self.dirty = False
 
@@ -201,11 +226,14 @@ class _ClassTable:
 class Engine:
     """Continuous batching of swept solves on one GPU."""
 
-    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None):
+    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
+                 poll_rounds=4, max_rounds_ahead=1 << 20):
         hip.ensure_init(0)
         self.hbm_budget = hbm_budget if hbm_budget is not None else 64 << 30
         self.max_active = max_active
         self.use_graphs = use_graphs
+        self.poll_rounds = poll_rounds
+        self.max_rounds_ahead = max_rounds_ahead
         self.cus = cus
         self.tuner = tuner
         self.classes = {}
@@ -241,13 +269,24 @@ class Engine:
                 break
         return self.finished
 
+    def _rounds_ahead(self):
+        """Rounds to launch before the host next needs to act: up to the
+        next retirement, and at most ``poll_rounds`` while requests that have
+        not arrived yet are queued (so they are admitted promptly)."""
+        r = min(t.c_end - t.c for t in self.running)
+        if self.queue:
+            r = min(r, self.poll_rounds)
+        return max(1, min(r, self.max_rounds_ahead))
+
     def step(self):
-        """Admit what fits, run one round of every class, retire finished."""
+        """Admit what fits, run rounds of every class up to the next
+        retirement (or admission check), retire finished problems."""
         self._admit()
         if not self.running:
             if self.queue:
                 time.sleep(max(0.0, min(t.arrival for t in self.queue) - self.now()))
             return
+        R = self._rounds_ahead()
         for key, tab in self.tables.items():
             if tab.dirty:
                 tab.rebuild(self.cus)
@@ -259,12 +298,13 @@ class Engine:
                     with g:
                         hip.multi_run(tab.plan, 1)
                     tab.graph = g
-                tab.graph.launch()
+                for _ in range(R):
+                    tab.graph.launch()
             else:
-                hip.multi_run(tab.plan, 1)
-        self.rounds += 1
+                hip.multi_run(tab.plan, R)
+        self.rounds += R
         for t in list(self.running):
-            t.c += 1
+            t.c += R
             if t.c >= t.c_end:
                 self._retire(t)
 
@@ -315,7 +355,12 @@ class Engine:
         if nv != cls.nv or nx % b or ny % b:
             raise ValueError(f"problem shape {p.ic.shape} does not fit class {cls.key}")
         N = schedule.requested_steps(p.globals[0], p.globals[1], p.globals[2])
-        if p.method == "standard":
+        method = p.method
+        if method == "auto":
+            method = (self.tuner.method(p.plugin, p.precision, p.its) if self.tuner
+                      else DEFAULT_METHOD.get((p.plugin, p.precision), "swept"))
+        t.method = method
+        if method == "standard":
             t.times = schedule.standard_times(N, b, cls.ops, cls.its)
         else:
             t.times = schedule.swept_times(N, b, cls.ops, cls.its, exact=p.exact)

@@ -29,6 +29,28 @@ def build_library():
     return OUT
 
 
+H5_SRC = os.path.join(HERE, "csrc", "h5io.c")
+H5_OUT = os.path.join(LIBDIR, "libpysweep_h5.so")
+H5_PREFIX = os.environ.get("PYSWEEP_HDF5_PREFIX", "/opt/conda")
+
+
+def build_h5():
+    """libpysweep_h5.so against the HDF5 C library when it is installed
+    (HDF5 output in the reference's layout); None otherwise."""
+    inc, lib = os.path.join(H5_PREFIX, "include"), os.path.join(H5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5.h")) or not os.path.exists(os.path.join(lib, "libhdf5.so")):
+        return None
+    os.makedirs(LIBDIR, exist_ok=True)
+    tmp = H5_OUT + ".tmp"
+    cmd = ["gcc", "-O2", "-fPIC", "-shared", "-Wall", "-I", inc, H5_SRC, "-o", tmp, "-L", lib, "-lhdf5",
+           "-Wl,-rpath," + lib]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("gcc failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    os.replace(tmp, H5_OUT)
+    return H5_OUT
+
+
 def precompile_plugins(verbose=False):
     """hiprtc-compile the kernel modules the tests, smoke() and bench.py use
     (no GPU needed) so the GPU host loads them from the cache."""
@@ -44,6 +66,7 @@ def precompile_plugins(verbose=False):
 
 def main():
     print("built", build_library(), flush=True)
+    print("built", build_h5() or "no HDF5 library found: .npz output", flush=True)
     n = precompile_plugins(verbose="-v" in sys.argv)
     print(f"pre-compiled {n} kernel modules", flush=True)
 

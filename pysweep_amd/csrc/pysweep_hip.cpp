@@ -479,12 +479,20 @@ int ps_copy_f64(void *dst, const void *src, size_t n)
     return 0;
 }
 
+__global__ void ps_owner_kernel(const int *__restrict__ prefix, int n, int *__restrict__ owner)
+{
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int lo = prefix[i], hi = prefix[i + 1];
+    for (int t = lo + threadIdx.x; t < hi; t += blockDim.x) owner[t] = i;
+}
+
 static int multi_round(const ps_multi_plan *p)
 {
-    void *insts = p->insts, *prefix = p->prefix, *sprefix = p->sprefix;
+    void *insts = p->insts, *prefix = p->prefix, *sprefix = p->sprefix, *owner = p->owner, *sowner = p->sowner;
     int n = p->ninst;
-    void *args[] = {&insts, &prefix, &n};
-    void *sargs[] = {&insts, &sprefix, &n};
+    void *args[] = {&insts, &prefix, &owner, &n};
+    void *sargs[] = {&insts, &sprefix, &sowner, &n};
     int rc = 0;
     if (p->total_tiles > 0) {
         const unsigned g = static_cast<unsigned>(p->grid > 0 && p->grid < p->total_tiles ? p->grid : p->total_tiles);
@@ -506,10 +514,31 @@ static int multi_round(const ps_multi_plan *p)
     return launch(p->advance, static_cast<unsigned>((n + 255) / 256), 256, aargs);
 }
 
+int ps_multi_index(const ps_multi_plan *plan)
+{
+    PS_NEED_INIT();
+    if (!plan || plan->ninst <= 0) return fail("empty multi plan");
+    if (plan->total_tiles > 0) {
+        if (!plan->owner || !plan->prefix) return fail("multi plan without swept owner table");
+        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, g_stream,
+                           static_cast<const int *>(plan->prefix), plan->ninst, static_cast<int *>(plan->owner));
+        PS_HIP(hipGetLastError());
+    }
+    if (plan->total_stiles > 0) {
+        if (!plan->sowner || !plan->sprefix) return fail("multi plan without standard owner table");
+        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, g_stream,
+                           static_cast<const int *>(plan->sprefix), plan->ninst, static_cast<int *>(plan->sowner));
+        PS_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
 int ps_multi_run(const ps_multi_plan *plan, int rounds)
 {
     PS_NEED_INIT();
     if (!plan || !plan->insts || !plan->prefix || !plan->advance) return fail("incomplete multi plan");
+    if ((plan->total_tiles > 0 && !plan->owner) || (plan->total_stiles > 0 && !plan->sowner))
+        return fail("multi plan without owner tables (ps_multi_index)");
     if (plan->total_tiles > 0 && (!plan->up || !plan->ybridge || !plan->xbridge || !plan->octahedron || !plan->down))
         return fail("multi plan without phase kernels");
     for (int r = 0; r < rounds; ++r) {

@@ -53,7 +53,8 @@ class Inst(ctypes.Structure):
 
 
 class MultiPlan(ctypes.Structure):
-    _fields_ = [("insts", c_void_p), ("prefix", c_void_p), ("sprefix", c_void_p), ("ninst", c_int),
+    _fields_ = [("insts", c_void_p), ("prefix", c_void_p), ("sprefix", c_void_p), ("owner", c_void_p),
+                ("sowner", c_void_p), ("ninst", c_int),
                 ("total_tiles", c_int), ("total_stiles", c_int), ("grid", c_int), ("sgrid", c_int),
                 ("threads", c_int), ("sthreads", c_int),
                 ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p), ("octahedron", c_void_p),
@@ -116,6 +117,7 @@ _standard_run_timed = _sig("ps_standard_run_timed", ctypes.POINTER(StandardPlan)
                            ctypes.POINTER(RunStats))
 _copy_f64 = _sig("ps_copy_f64", c_void_p, c_void_p, c_size_t)
 _multi_run = _sig("ps_multi_run", ctypes.POINTER(MultiPlan), c_int)
+_multi_index = _sig("ps_multi_index", ctypes.POINTER(MultiPlan))
 _multi_run_timed = _sig("ps_multi_run_timed", ctypes.POINTER(MultiPlan), c_int, ctypes.POINTER(RunStats))
 _graph_begin = _sig("ps_graph_begin")
 _graph_end = _sig("ps_graph_end", ctypes.POINTER(c_void_p))
@@ -129,7 +131,7 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_launch", "ps_synchronize",
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_destroy",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
-           "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
+           "ps_multi_index", "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
 
 
@@ -296,6 +298,11 @@ def swept_run_timed(plan, G, c_begin, c_end, stats):
 
 def standard_run_timed(plan, g_begin, g_end, stats):
     check(_standard_run_timed(ctypes.byref(plan), g_begin, g_end, ctypes.byref(stats)), "ps_standard_run_timed")
+
+
+def multi_index(plan):
+    """Fill the plan's owner tables from its prefix sums (on the work stream)."""
+    check(_multi_index(ctypes.byref(plan)), "ps_multi_index")
 
 
 def multi_run(plan, rounds, stats=None):

@@ -118,3 +118,18 @@ def test_fp32_state_error_against_fp64_oracle(device):
     assert lin_h <= 1e-6 and l2_h <= 1e-6
     assert np.isfinite(r[1].result).all()
     assert l2_e <= 1e-3
+
+
+def test_auto_method_and_round_batching_match_oracle(device):
+    """method="auto" (the faster kernel per class) and engines that run one
+    round per host step or many give the oracle's results bitwise."""
+    mix = _mix()
+    for p, _ in mix:
+        p.method = "auto"
+    a, eng = _run([p for p, _ in mix])
+    b, _ = _run([p for p, _ in mix], max_rounds_ahead=1)
+    for t, u, (p, step) in zip(a, b, mix):
+        steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
+        want = oracle_stages(step, p.ic, p.its, p.its * steps)
+        assert np.array_equal(t.result, want) and np.array_equal(u.result, want)
+    assert {t.method for t in a} <= {"swept", "standard"}
