@@ -178,9 +178,10 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  * per-cell step's order from the same helper functions, so the results are
  * bitwise those of step().
  *
- *   s      LDS tile, level r-1 at s[base + v*VARS + x*SY + y]; level r is
- *          written at base + TIMES, the base of a final stage read at
- *          base - TIMES;
+ *   cur    level r-1 in LDS, variable v of cell (x, y) at
+ *          cur[v*VARS + x*SY + y]; prev is level r-2 (the base of a final
+ *          stage) and out receives level r, with the same strides (three
+ *          separate slots, so the caller may keep its window circular);
  *   sc     scratch of PS_LEVEL_SCRATCH(SX * SY, PS_MAXFACES) doubles,
  *          PS_MAXFACES being the most faces of one direction of any
  *          rectangle the harness passes ((x1-x0+1)*(y1-y0) or
@@ -191,11 +192,25 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
 #define PS_HAS_LEVEL 1
 #define PS_LEVEL_SCRATCH(CELLS, FACES) ((CELLS) + 8 * (FACES))
 
+/* t / w for 0 <= t < 2^22, w >= 1, rw = 1.0f / w: the float quotient is
+ * within one of the true one; one correction makes it exact */
+__device__ __forceinline__ int eu_div(int t, int w, float rw)
+{
+    int q = (int)((float)t * rw);
+    const int r = t - q * w;
+    if (r < 0)
+        q -= 1;
+    else if (r >= w)
+        q += 1;
+    return q;
+}
+
 template <class R, int MAXC>
-__device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0, int y1, int globalTimeStep,
-                                         double *sc)
+__device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
+                                         int globalTimeStep, double *sc)
 {
     const int h = x1 - x0, w = y1 - y0;
+    const float rw = 1.0f / (float)w;
     constexpr int FS = PS_MAXFACES;  /* component stride of the face buffer */
     double *P = sc;                  /* pressures, tile-indexed x*SY + y */
     double *F = sc + SX * SY;        /* face f, component c at F[c*FS + f]: flux 0-3, dissipation 4-7 */
@@ -203,12 +218,13 @@ __device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0,
     /* 1. pressures on the cross-shaped footprint */
     {
         const int bh = h + 4, bw = w + 4, n = bh * bw;
+        const float rbw = 1.0f / (float)bw;
         for (int t = threadIdx.x; t < n; t += PS_NT) {
-            const int i = t / bw, j = t - (t / bw) * bw;
+            const int i = eu_div(t, bw, rbw), j = t - i * bw;
             const bool inx = i >= 2 && i < h + 2, iny = j >= 2 && j < w + 2;
             if (!(inx || iny)) continue;
             const int x = x0 - 2 + i, y = y0 - 2 + j;
-            P[x * SY + y] = eu_pressure(eu_at(s, base + x * SY + y));
+            P[x * SY + y] = eu_pressure(eu_at(cur, x * SY + y));
         }
     }
     __syncthreads();
@@ -217,10 +233,10 @@ __device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0,
     {
         const int n = (h + 1) * w;
         for (int t = threadIdx.x; t < n; t += PS_NT) {
-            const int i = t / w, j = t - (t / w) * w;
+            const int i = eu_div(t, w, rw), j = t - i * w;
             const int x = x0 + i, y = y0 + j;
-            const int idx = base + x * SY + y;
-            const eu_q wq = eu_at(s, idx - SY), c = eu_at(s, idx);
+            const int idx = x * SY + y;
+            const eu_q wq = eu_at(cur, idx - SY), c = eu_at(cur, idx);
             const double pww = P[(x - 2) * SY + y], pw = P[(x - 1) * SY + y];
             const double pc = P[x * SY + y], pe = P[(x + 1) * SY + y];
             const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
@@ -239,7 +255,7 @@ __device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0,
     for (int m = 0; m < MAXC; ++m) {
         const int t = threadIdx.x + m * PS_NT;
         if (t < h * w) {
-            const int i = t / w, j = t - (t / w) * w;
+            const int i = eu_div(t, w, rw), j = t - i * w;
             const double *fw = F + (i * w + j), *fe = F + ((i + 1) * w + j);
             eu_q a;
             a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
@@ -256,11 +272,12 @@ __device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0,
     /* 4. y faces: face j (0..w) lies south of cell y0+j */
     {
         const int wf = w + 1, n = h * wf;
+        const float rwf = 1.0f / (float)wf;
         for (int t = threadIdx.x; t < n; t += PS_NT) {
-            const int i = t / wf, j = t - (t / wf) * wf;
+            const int i = eu_div(t, wf, rwf), j = t - i * wf;
             const int x = x0 + i, y = y0 + j;
-            const int idx = base + x * SY + y;
-            const eu_q wq = eu_at(s, idx - 1), c = eu_at(s, idx);
+            const int idx = x * SY + y;
+            const eu_q wq = eu_at(cur, idx - 1), c = eu_at(cur, idx);
             const double pww = P[x * SY + y - 2], pw = P[x * SY + y - 1];
             const double pc = P[x * SY + y], pe = P[x * SY + y + 1];
             const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
@@ -280,7 +297,7 @@ __device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0,
     for (int m = 0; m < MAXC; ++m) {
         const int t = threadIdx.x + m * PS_NT;
         if (t < h * w) {
-            const int i = t / w, j = t - (t / w) * w;
+            const int i = eu_div(t, w, rw), j = t - i * w;
             const double *fs = F + (i * (w + 1) + j), *fn = fs + 1;
             eu_q a;
             a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
@@ -290,19 +307,19 @@ __device__ __forceinline__ void ps_level(R *s, int base, int x0, int x1, int y0,
             a.r = a.r - fn[4 * FS]; a.m = a.m - fn[5 * FS]; a.n = a.n - fn[6 * FS]; a.e = a.e - fn[7 * FS];
             a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
             const eu_q fx = dfx[m];
-            const int idx = base + (x0 + i) * SY + (y0 + j);
-            const eu_q b = eu_at(s, fin ? idx - TIMES : idx);
-            R *out = s + idx + TIMES;
+            const int idx = (x0 + i) * SY + (y0 + j);
+            const eu_q b = eu_at(fin ? prev : cur, idx);
+            R *o = out + idx;
             if (fin) {
-                out[0] = (R)(b.r + (fx.r * DTDX + a.r * DTDY));
-                out[VARS] = (R)(b.m + (fx.m * DTDX + a.m * DTDY));
-                out[2 * VARS] = (R)(b.n + (fx.n * DTDX + a.n * DTDY));
-                out[3 * VARS] = (R)(b.e + (fx.e * DTDX + a.e * DTDY));
+                o[0] = (R)(b.r + (fx.r * DTDX + a.r * DTDY));
+                o[VARS] = (R)(b.m + (fx.m * DTDX + a.m * DTDY));
+                o[2 * VARS] = (R)(b.n + (fx.n * DTDX + a.n * DTDY));
+                o[3 * VARS] = (R)(b.e + (fx.e * DTDX + a.e * DTDY));
             } else {
-                out[0] = (R)(b.r + hh * (fx.r * DTDX + a.r * DTDY));
-                out[VARS] = (R)(b.m + hh * (fx.m * DTDX + a.m * DTDY));
-                out[2 * VARS] = (R)(b.n + hh * (fx.n * DTDX + a.n * DTDY));
-                out[3 * VARS] = (R)(b.e + hh * (fx.e * DTDX + a.e * DTDY));
+                o[0] = (R)(b.r + hh * (fx.r * DTDX + a.r * DTDY));
+                o[VARS] = (R)(b.m + hh * (fx.m * DTDX + a.m * DTDY));
+                o[2 * VARS] = (R)(b.n + hh * (fx.n * DTDX + a.n * DTDY));
+                o[3 * VARS] = (R)(b.e + hh * (fx.e * DTDX + a.e * DTDY));
             }
         }
     }

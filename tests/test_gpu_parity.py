@@ -207,3 +207,51 @@ def test_config_euler_1024_full_size(device):
     ex = o_euler.vortex(X, Y, steps * g[2], 1.4)
     err = np.abs(sw.final_state[0] - ex[0]).max()
     assert err < 0.05
+
+
+def test_standard_ragged_tiles(device):
+    """Grid sides that are not multiples of the standard tile (partial tiles
+    at the high edges)."""
+    nx, ny, steps = 40, 24, 5
+    dx = 10.0 / nx
+    dt = 0.1 * dx
+    X, Y = np.meshgrid(o_euler.grid(nx), np.linspace(-5, 5, ny, endpoint=False), indexing="ij")
+    ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=2.5)
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    sol = gpu_solve("euler", ic, g, 2, 2, 8, swept=False, exact=True)
+    want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
+    assert np.array_equal(sol.final_state, want)
+
+
+@pytest.mark.parametrize("b", [32, 64])
+def test_large_heat_swept_equals_standard_and_decays(device, b):
+    """Heat 8192^2 (the large-instance size), 200 forward-Euler steps:
+    swept == standard bitwise; the sin(2 pi x) sin(2 pi y) mode decays as
+    exp(-8 pi^2 t) to O(dx^2)."""
+    from pysweep_amd import workloads
+    n, steps = 8192, 200
+    dx = 1.0 / n
+    dt = 0.1 * dx * dx
+    ic = workloads.heat_mode(n, n)
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
+    sw = gpu_solve("heat", ic, g, 1, 1, b, swept=True, exact=True)
+    st = gpu_solve("heat", ic, g, 1, 1, b, swept=False, exact=True)
+    assert np.array_equal(sw.final_state, st.final_state)
+    ratio = sw.final_state[0, n // 4, n // 4] / ic[0, n // 4, n // 4]
+    assert abs(ratio - np.exp(-8 * np.pi ** 2 * steps * dt)) < 1e-9
+
+
+def test_large_euler_swept_equals_standard_conserves(device):
+    """Euler 2048^2, b=32, 100 steps: swept == standard bitwise and the four
+    integrals conserved to round-off."""
+    from pysweep_amd import workloads
+    n, steps = 2048, 100
+    dx = 10.0 / n
+    dt = 0.1 * dx
+    ic = workloads.vortex_state(n, n)
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    sw = gpu_solve("euler", ic, g, 2, 2, 32, swept=True, exact=True)
+    st = gpu_solve("euler", ic, g, 2, 2, 32, swept=False, exact=True)
+    assert np.array_equal(sw.final_state, st.final_state)
+    for v in range(4):
+        assert abs(sw.final_state[v].sum() - ic[v].sum()) <= 1e-11 * np.abs(ic[v]).sum()
