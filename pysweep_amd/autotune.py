@@ -1,5 +1,5 @@
-"""Autotuner for instance-table classes: block size, LDS window, workgroup
-size and persistent blocks per CU.
+"""Autotuner for instance-table classes: method (swept or standard), block
+size, LDS window, workgroup size and persistent blocks per CU.
 
 ``Tuner.tune`` runs a small sample batch with each candidate launch shape
 and keeps the fastest (cell-steps per second); every candidate computes the
@@ -26,8 +26,9 @@ DEFAULTS = {
 }
 
 CANDIDATES = {
-    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "grid_per_cu": 8}, {"b": 32, "grid_per_cu": 4}],
-    "heat": [{"b": 16}, {"b": 32}, {"b": 32, "grid_per_cu": 8}, {"b": 64}],
+    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "grid_per_cu": 8}, {"b": 32, "grid_per_cu": 4},
+              {"b": 16, "method": "standard"}],
+    "heat": [{"b": 16}, {"b": 32}, {"b": 32, "grid_per_cu": 8}, {"b": 64}, {"b": 32, "method": "standard"}],
 }
 
 
@@ -74,34 +75,41 @@ class Tuner:
     def tune(self, plugin, precision, its, make_problems, candidates=None, save=True):
         """Time each candidate on ``make_problems()`` (a list of Problems);
         return and store the fastest."""
-        from pysweep_amd import batch
+        from pysweep_amd import batch, hip
         trials = []
         for cand in candidates or CANDIDATES[plugin]:
             probs = make_problems()
             for p in probs:
                 p.b = cand["b"]
+                p.method = cand.get("method", "swept")
 
             class _Fixed:
                 def block_size(self_, *a):
                     return cand["b"]
 
                 def config(self_, *a):
-                    return {k: v for k, v in cand.items() if k != "b"}
+                    return {k: v for k, v in cand.items() if k not in ("b", "method")}
 
-            eng = batch.Engine(tuner=_Fixed())
+            eng = batch.Engine(tuner=_Fixed(), max_rounds_ahead=1)
             for p in probs:
                 eng.submit(p, arrival=0.0)
             eng.step()  # admit + compile + first round (not timed)
+            hip.synchronize()
+            eng.max_rounds_ahead = 1 << 20
             t0 = time.perf_counter()
             eng.run()
+            hip.synchronize()
             sec = time.perf_counter() - t0
             cells = 0
             for t in eng.finished:
                 nv, nx, ny = t.problem.ic.shape
                 cells += nx * ny * (t.times.stages + t.times.tail) / t.problem.its
-            trials.append({**cand, "seconds": sec, "gcups": cells / sec / 1e9})
+            trials.append({"method": "swept", **cand, "seconds": sec, "gcups": cells / sec / 1e9,
+                           "results": {t.problem.tag: t.result for t in eng.finished}})
         best = max(trials, key=lambda r: r["gcups"])
-        rec = {k: v for k, v in best.items() if k in ("b", "slots", "threads", "grid_per_cu")}
+        self.last_trials = trials
+        rec = {k: v for k, v in best.items() if k in ("b", "slots", "threads", "grid_per_cu", "method")}
+        trials = [{k: v for k, v in t.items() if k != "results"} for t in trials]
         rec["trials"] = trials
         self.records[_key(plugin, precision, its)] = rec
         if save and self.path:

@@ -46,3 +46,49 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
 {
     heat_step(shared_state, idx, globalTimeStep);
 }
+
+/*
+ * Level update used by the pysweep_amd kernels (PS_HAS_LEVEL): the cells of
+ * [x0,x1) x [y0,y1) of one level, lane t taking cells t, t + PS_NT, ...
+ * (row-major, y fastest).  cur is level r-1, prev level r-2 (the base of a
+ * final RK2 stage), out level r, all with the tile strides; the arithmetic is
+ * heat_step's, so results are bitwise the same.  Separate level pointers let
+ * the swept kernels keep their LDS window circular (no copies).
+ */
+#define PS_HAS_LEVEL 1
+#define PS_LEVEL_SCRATCH(CELLS, FACES) 1
+/* the standard kernel keeps the per-cell step (one pass over the tile,
+ * measured faster there) */
+#define PS_LEVEL_SWEPT_ONLY 1
+
+/* t / w for 0 <= t < 2^22, w >= 1, rw = 1.0f / w (one correction) */
+__device__ __forceinline__ int heat_div(int t, int w, float rw)
+{
+    int q = (int)((float)t * rw);
+    const int r = t - q * w;
+    if (r < 0)
+        q -= 1;
+    else if (r >= w)
+        q += 1;
+    return q;
+}
+
+template <class R, int MAXC>
+__device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
+                                         int globalTimeStep, double *)
+{
+    const int w = y1 - y0, n = (x1 - x0) * w;
+    const float rw = 1.0f / (float)w;
+    const bool fin = globalTimeStep % 2 == 0;
+    for (int t = threadIdx.x; t < n; t += PS_NT) {
+        const int i = heat_div(t, w, rw);
+        const int idx = (x0 + i) * SY + y0 + (t - i * w);
+        const double d = heat_laplace(cur, idx);
+        if (SCHEME)
+            out[idx] = (R)(d + (double)cur[idx]);
+        else if (fin)
+            out[idx] = (R)(1.0 * d + (double)prev[idx]);
+        else
+            out[idx] = (R)(0.5 * d + (double)cur[idx]);
+    }
+}

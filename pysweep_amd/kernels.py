@@ -83,7 +83,8 @@ class ModuleSpec:
             return tile + level_scratch_bytes(self.user_path, self.generic, self.b, self.b, swept_faces(self.b, self.ops))
         sx, sy = self.bx + 2 * self.ops, self.by + 2 * self.ops
         tile = (self.its + 1) * self.nv * sx * sy * self.real_bytes
-        return tile + level_scratch_bytes(self.user_path, self.generic, sx, sy, (self.bx + 1) * self.by)
+        return tile + level_scratch_bytes(self.user_path, self.generic, sx, sy,
+                                          max((self.bx + 1) * self.by, self.bx * (self.by + 1)), "standard")
 
     def describe(self):
         g = (f"b={self.b} slots={self.slots}" if self.kind == "swept" else f"tile={self.bx}x{self.by}")
@@ -103,11 +104,36 @@ def swept_faces(b, ops):
     return (m + 1) * m
 
 
-def level_scratch_bytes(user_path, generic, sx, sy, faces):
-    """LDS of a plugin's level update: a tile of pressures and 8 doubles per face."""
-    if generic or not has_level_update(user_path):
-        return 0
-    return (sx * sy + 8 * faces) * 8
+_SCRATCH_RE = re.compile(r"^[ \t]*#[ \t]*define[ \t]+PS_LEVEL_SCRATCH\((\w+),\s*(\w+)\)(.*)$", re.M)
+
+
+def level_scratch(user_path):
+    """The plugin's PS_LEVEL_SCRATCH(cells, faces) as a Python function
+    (doubles of LDS scratch), or None without a level update."""
+    if not has_level_update(user_path):
+        return None
+    m = _SCRATCH_RE.search(read_plugin_source(user_path))
+    if m is None:
+        return lambda cells, faces: 0
+    a, b, expr = m.group(1), m.group(2), m.group(3).strip()
+    if not re.fullmatch(r"[\w\s()+*/-]*", expr):
+        raise ValueError(f"PS_LEVEL_SCRATCH must be integer arithmetic of {a} and {b}: {expr!r}")
+    code = compile(expr.replace("/", "//"), "PS_LEVEL_SCRATCH", "eval")
+    return lambda cells, faces: int(eval(code, {"__builtins__": {}}, {a: cells, b: faces}))
+
+
+def level_update_in_standard(user_path):
+    """False when the plugin asks the standard kernel to keep its per-cell
+    step (PS_LEVEL_SWEPT_ONLY)."""
+    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_LEVEL_SWEPT_ONLY\b", read_plugin_source(user_path),
+                     re.M) is None
+
+
+def level_scratch_bytes(user_path, generic, sx, sy, faces, kind="swept"):
+    """LDS bytes of a plugin's level update for an sx x sy tile."""
+    f = None if generic or (kind == "standard" and not level_update_in_standard(user_path)) \
+        else level_scratch(user_path)
+    return 0 if f is None else 8 * f(sx * sy, faces)
 
 
 def default_slots(nv, its, b, P, real_bytes=8, scratch=0):
@@ -149,30 +175,35 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
                       real=real, multi=multi, generic=generic)
 
 
-def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, level_update=False, threads=256):
+def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scratch=None, threads=256):
     """Tile (bx rows, by columns) for the standard kernel: most cells within
     the LDS budget, then rows of at least 32 doubles (coalesced wavefront
-    loads), then the least halo re-reading.
+    loads), then the least halo re-reading.  ``scratch`` is the plugin's
+    level-update scratch function (``level_scratch``) or None.
 
-    With a plugin's cooperative level update every pass of the update (the
-    cells' pressures on the cross-shaped footprint, the x faces, the y faces)
-    must fit in one sweep of the workgroup's lanes: a second, mostly idle
-    sweep costs as much as the first.  Measured on MI355X (Euler 1024^2,
-    256 lanes): 8x16 7.1 GCUPS, 8x32 6.6, 16x16 6.5."""
+    When the level update shares faces through LDS (its scratch grows with
+    the faces), every pass of the update (the cells' pressures on the
+    cross-shaped footprint, the x faces, the y faces) must fit in one sweep
+    of the workgroup's lanes: a second, mostly idle sweep costs as much as
+    the first.  Measured on MI355X (Euler 1024^2, 256 lanes): 8x16 7.1
+    GCUPS, 8x32 6.6, 16x16 6.5."""
+    faces_shared = scratch is not None and scratch(0, 1) > scratch(0, 0)
     best = None
     for by in (64, 32, 16, 8):
         for bx in (32, 16, 8, 4, 2):
             if bx > nx or by > ny:
                 continue
-            lds = (its + 1) * nv * (bx + 2 * ops) * (by + 2 * ops) * real_bytes
-            if level_update:
-                lds += ((bx + 2 * ops) * (by + 2 * ops) + 8 * (bx + 1) * by) * 8
-                foot = (bx + 2 * ops) * (by + 2 * ops) - 4 * ops * ops
+            sx, sy = bx + 2 * ops, by + 2 * ops
+            lds = (its + 1) * nv * sx * sy * real_bytes
+            if scratch is not None:
+                lds += 8 * scratch(sx * sy, max((bx + 1) * by, bx * (by + 1)))
+            if faces_shared:
+                foot = sx * sy - 4 * ops * ops
                 if max((bx + 1) * by, bx * (by + 1), foot) > threads:
                     continue
             if lds > lds_budget:
                 continue
-            halo = (bx + 2 * ops) * (by + 2 * ops) / (bx * by)
+            halo = sx * sy / (bx * by)
             score = (bx * by, by >= 32, -halo)
             if best is None or score > best[0]:
                 best = (score, bx, by)
@@ -184,8 +215,8 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, lev
 def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, lds_budget=LDS_BUDGET,
                   real="double", multi=False, mpss=1, generic=False):
     if bx is None or by is None:
-        lu = not generic and has_level_update(user_path)
-        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, lu, threads)
+        sc = None if generic or not level_update_in_standard(user_path) else level_scratch(user_path)
+        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, sc, threads)
     if bx > nx or by > ny:
         raise ValueError("standard tile larger than the grid")
     return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads,

@@ -133,3 +133,34 @@ def test_auto_method_and_round_batching_match_oracle(device):
         want = oracle_stages(step, p.ic, p.its, p.its * steps)
         assert np.array_equal(t.result, want) and np.array_equal(u.result, want)
     assert {t.method for t in a} <= {"swept", "standard"}
+
+
+def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(device, tmp_path):
+    """Tuner.tune times each candidate (method, block size, persistent grid)
+    on a sample batch, stores the fastest, and every candidate gives the
+    same bits."""
+    import json
+
+    from pysweep_amd.autotune import Tuner
+    path = str(tmp_path / "tuning.json")
+    tu = Tuner(path=path)
+
+    def make():
+        out = []
+        for i, (nx, ny, kx) in enumerate(((64, 64, 1), (128, 64, 2), (64, 128, 3))):
+            p, _ = _heat_problem(nx, ny, 24, kx=kx)
+            p.tag = i
+            out.append(p)
+        return out
+
+    cands = [{"b": 16}, {"b": 32}, {"b": 32, "grid_per_cu": 2}, {"b": 32, "method": "standard"}]
+    rec = tu.tune("heat", "fp64", 1, make, candidates=cands)
+    assert rec["b"] in (16, 32) and rec["method"] in ("swept", "standard")
+    with open(path) as f:
+        stored = json.load(f)["heat/fp64/its1"]
+    assert stored["b"] == rec["b"] and len(stored["trials"]) == len(cands)
+    first = tu.last_trials[0]["results"]
+    for t in tu.last_trials[1:]:
+        for tag, r in t["results"].items():
+            assert np.array_equal(r, first[tag]), (t, tag)
+    assert Tuner(path=path).method("heat", "fp64", 1) == rec["method"]
