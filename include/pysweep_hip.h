@@ -95,6 +95,9 @@ int ps_synchronize(void);
 int ps_event_create(ps_event *event);
 int ps_event_record(ps_event event);
 int ps_event_elapsed_ms(ps_event start, ps_event stop, float *ms);
+/* Block the host until the work recorded before the event is done (bounds
+ * how far a host loop runs ahead of the GPU). */
+int ps_event_synchronize(ps_event event);
 int ps_event_destroy(ps_event event);
 
 /* ---------------------------------------------------------------------- */

@@ -367,6 +367,12 @@ int ps_event_elapsed_ms(ps_event start, ps_event stop, float *ms)
     return 0;
 }
 
+int ps_event_synchronize(ps_event event)
+{
+    PS_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return 0;
+}
+
 int ps_event_destroy(ps_event event)
 {
     if (event) PS_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));

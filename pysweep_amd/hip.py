@@ -109,6 +109,7 @@ _event_create = _sig("ps_event_create", ctypes.POINTER(c_void_p))
 _event_record = _sig("ps_event_record", c_void_p)
 _event_elapsed = _sig("ps_event_elapsed_ms", c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float))
 _event_destroy = _sig("ps_event_destroy", c_void_p)
+_event_sync = _sig("ps_event_synchronize", c_void_p)
 _swept_run = _sig("ps_swept_run", ctypes.POINTER(SweptPlan), c_int, c_int, c_int)
 _standard_run = _sig("ps_standard_run", ctypes.POINTER(StandardPlan), c_int, c_int)
 _swept_run_timed = _sig("ps_swept_run_timed", ctypes.POINTER(SweptPlan), c_int, c_int, c_int,
@@ -129,7 +130,7 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_free_host", "ps_module_load", "ps_module_unload", "ps_module_get_global",
            "ps_module_get_function", "ps_function_info", "ps_malloc", "ps_free", "ps_memcpy_htod",
            "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_launch", "ps_synchronize",
-           "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_destroy",
+           "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_destroy",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
            "ps_multi_index", "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
@@ -325,6 +326,9 @@ class Event:
 
     def record(self):
         check(_event_record(self.h), "ps_event_record")
+
+    def synchronize(self):
+        check(_event_sync(self.h), "ps_event_synchronize")
 
     def elapsed_ms(self, stop):
         ms = ctypes.c_float()

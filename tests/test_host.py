@@ -265,3 +265,38 @@ def test_prefetch_register_bound_covers_every_face_load():
                 need[x0:x1, y0:y1] = False
                 most = max(most, int(need.sum()))
         assert compiler._face_cells(b, k, s) == most, (b, k, s)
+
+
+def test_short_first_selection():
+    """The engine's round selection (pysweep_amd.batch.short_first): the
+    shortest remaining work first, packed up to the capacity in cells; a
+    problem left out for max_wait seconds joins every round from then on.
+    The remaining-work arithmetic follows the round structure (swept rounds
+    cover (G+1)P stages, then single standard stages)."""
+    from pysweep_amd import batch, schedule
+
+    def ticket(times, n, c=0):
+        t = batch.Ticket(problem=None, times=times, cells=n * n, c=c)
+        t.c_end = times.G + 2 + times.tail
+        return t
+
+    big = ticket(schedule.swept_times(1000, 16, 2, 2, exact=True), 2048)
+    small = ticket(schedule.standard_times(100, 16, 2, 2), 128, c=1)
+    twin = ticket(schedule.standard_times(150, 16, 2, 2), 128, c=1)
+    assert small.remaining_work == 200 * 128 * 128
+    assert big.remaining_work == (big.times.stages + big.times.tail) * 2048 * 2048
+    big.c = big.times.G + 1  # at LastPrism
+    assert big.remaining_work == (big.times.P / 2 + big.times.tail) * 2048 * 2048
+    big.c = 0
+    now = 10.0
+    for t in (big, small, twin):
+        t.last_active = now
+    assert batch.short_first([big, small, twin], now) == {small, twin}
+    assert batch.short_first([big, small, twin], now + 1.0) == {small, twin}
+    # big has now sat out for max_wait: it joins, and stays in
+    assert batch.short_first([big, small, twin], now + 2.5) == {small, twin, big}
+    assert big.boosted and batch.short_first([big, small, twin], now + 2.6) == {small, twin, big}
+    # alone, the long problem always runs
+    lone = ticket(schedule.swept_times(1000, 16, 2, 2, exact=True), 2048)
+    assert batch.short_first([lone], now) == {lone}
+    assert batch.short_first([], now) == set()
