@@ -190,7 +190,7 @@ typedef struct ps_inst {
     int c;               /* round counter: 0 FirstPrism, 1..G UpPrism,        */
                          /* G+1 LastPrism, G+2.. tail stages                 */
     int active;
-    int pad_;
+    int tpi;             /* standard tiles per work item (>= 1)           */
 } ps_inst;
 
 typedef struct ps_multi_plan {

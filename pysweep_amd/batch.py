@@ -34,6 +34,16 @@ PLUGINS = {
 }
 
 
+def tiles_per_item(tiles, slots=2048, most=4):
+    """Consecutive standard tiles per work item of a batched standard
+    launch: a work item's table lookups and constant staging (three
+    dependent loads) are paid once per item -- at one tile per item they
+    cost a solo 2048^2 Euler solve 19 % against the plain kernel -- but a
+    problem needs enough items to fill the GPU (``slots`` ~ 2 workgroups per
+    SIMD slot of 256 CUs), so only large problems take up to ``most``."""
+    return max(1, min(most, tiles // slots))
+
+
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
 #: side (Euler FP64 1024^2: standard 7.1 GCUPS, swept b=16 4.7; heat 8192^2
@@ -188,21 +198,11 @@ class _ClassTable:
         self.plan = None
         self.graph = None
         self.owner = self.sowner = None
-        self.arr = None
         self.flags = None
 
-    def push_flags(self, active):
-        """Upload the problems' run flags (and their current round counters)
-        when the set of running problems that take part in rounds changes;
-        the captured graph stays valid (it reads the table at run time)."""
-        flags = tuple(t in active for t in self.running)
-        if flags == self.flags:
-            return
-        for i, t in enumerate(self.running):
-            self.arr[i].active = int(flags[i])
-            self.arr[i].c = t.c
-        hip.check(hip._htod(self.insts.ptr, ctypes.addressof(self.arr), ctypes.sizeof(self.arr)), "ps_memcpy_htod")
-        self.flags = flags
+    def needs(self, active):
+        """True when the table's run flags differ from ``active``."""
+        return self.flags != tuple(t in active for t in self.running)
 
     def rebuild(self, cus, active=None):
         cls = self.cls
@@ -211,6 +211,7 @@ class _ClassTable:
             self.plan = None
             self.graph = None
             self.dirty = False
+            self.flags = ()
             return
         if n > self.cap:
             self.cap = max(8, 2 * n)
@@ -229,11 +230,14 @@ class _ClassTable:
             st.stiles_x, st.stiles_y = -(-t.nx // sbx), -(-t.ny // sby)
             st.G, st.tail, st.c = t.times.G, t.times.tail, t.c
             st.active = int(active is None or t in active)
-            # problems without swept cycles (G = -1) take no swept work items
-            pre[i + 1] = pre[i] + (st.tiles_x * st.tiles_y if t.times.G >= 0 else 0)
-            spre[i + 1] = spre[i] + (st.stiles_x * st.stiles_y if t.times.tail else 0)
+            st.tpi = tiles_per_item(st.stiles_x * st.stiles_y)
+            # problems without swept cycles (G = -1) take no swept work items,
+            # problems sitting out of the rounds none at all
+            run = bool(st.active)
+            pre[i + 1] = pre[i] + (st.tiles_x * st.tiles_y if t.times.G >= 0 and run else 0)
+            items = -(-st.stiles_x * st.stiles_y // st.tpi)
+            spre[i + 1] = spre[i] + (items if t.times.tail and run else 0)
         hip.check(hip._htod(self.insts.ptr, ctypes.addressof(arr), ctypes.sizeof(arr)), "ps_memcpy_htod")
-        self.arr = arr
         self.flags = tuple(bool(arr[i].active) for i in range(n))
         self.prefix.upload(pre)
         self.sprefix.upload(spre)
@@ -372,12 +376,9 @@ class Engine:
             active = set(self.running)
         R = self._rounds_ahead(active)
         for key, tab in self.tables.items():
-            if tab.dirty:
+            if tab.dirty or tab.needs(active):
                 tab.rebuild(self.cus, active)
-            if tab.plan is None:
-                continue
-            tab.push_flags(active)
-            if not any(tab.flags):
+            if tab.plan is None or not any(tab.flags):
                 continue
             if self.use_graphs:
                 if tab.graph is None:

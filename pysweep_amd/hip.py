@@ -49,7 +49,7 @@ class Inst(ctypes.Structure):
     """ps_inst: one problem of an instance-table launch."""
     _fields_ = [("ring", c_void_p), ("consts", c_void_p), ("nx", c_int), ("ny", c_int), ("T", c_int),
                 ("tiles_x", c_int), ("tiles_y", c_int), ("stiles_x", c_int), ("stiles_y", c_int),
-                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int), ("pad_", c_int)]
+                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int), ("tpi", c_int)]
 
 
 class MultiPlan(ctypes.Structure):
