@@ -104,14 +104,25 @@ def class_specs(plugin, precision, its, b, slots=None, threads=None):
     return spec, sspec
 
 
+def solo_specs(plugin, precision, its, b):
+    """The plain modules a class uses for a problem running alone."""
+    py, cu64, cu32, nv, ops, _ = PLUGINS[plugin]
+    real = "double" if precision == "fp64" else "float"
+    cu = eq_path(cu64 if precision == "fp64" else cu32)
+    return [kernels.swept_spec(cu, nv, ops, its, b, real=real),
+            kernels.standard_spec(cu, nv, ops, its, 1 << 16, 1 << 16, real=real)]
+
+
 def shipped_class_specs():
-    """Instance-table modules used by the tests and benchmarks."""
+    """Instance-table modules (and their one-problem modules) used by the
+    tests and benchmarks."""
     out = []
     for plugin, precision, its, b in (("euler", "fp64", 2, 8), ("euler", "fp64", 2, 16), ("euler", "fp64", 2, 32),
                                       ("euler", "fp32", 2, 16), ("euler", "fp32", 2, 32),
                                       ("heat", "fp64", 1, 16), ("heat", "fp64", 1, 32), ("heat", "fp64", 2, 16),
                                       ("heat", "fp64", 1, 64), ("heat", "fp32", 1, 16), ("heat", "fp32", 1, 32)):
         out.extend(class_specs(plugin, precision, its, b))
+        out.extend(solo_specs(plugin, precision, its, b))
     return out
 
 
@@ -135,9 +146,21 @@ class PluginClass:
         self.cpu = plugins.load_cpu_module(eq_path(py), name=f"pysweep_batch_{plugin}")
         self.grid_per_cu = grid_per_cu
         self.key = (plugin, precision, its, b)
+        self._solo = None
+        self.solo_consts = None  # problem whose constants the solo modules hold
 
     def ring_bytes(self, nx, ny):
         return (2 * self.P + self.its) * self.nv * nx * ny * self.rb
+
+    def solo_modules(self):
+        """The class's one-problem modules (the plain swept phases and the
+        standard kernel), for a problem that runs alone."""
+        if self._solo is None:
+            sw, st = solo_specs(self.plugin, self.precision, self.its, self.b)
+            msw, mst = kernels.load(sw), kernels.load(st)
+            f = {n: msw.get_function(n) for n in kernels.SWEPT_KERNELS}
+            self._solo = (sw, st, msw, mst, f, mst.get_function("Standard"))
+        return self._solo
 
 
 @dataclasses.dataclass(eq=False)
@@ -294,9 +317,10 @@ class Engine:
     latency, so it is not the default)."""
 
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
-                 poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2):
+                 poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True):
         hip.ensure_init(0)
         self.policy = policy
+        self.solo = solo  # a problem running alone uses the plain kernels
         # host steps whose rounds may be queued on the GPU at once: admitting
         # a request waits for the queued rounds, so this bounds its delay
         self.max_inflight = max_inflight
@@ -362,6 +386,48 @@ class Engine:
                 r = min(r, max(self.poll_rounds, fit))
         return max(1, min(r, self.max_rounds_ahead))
 
+    def _run_solo(self, t, R):
+        """Rounds c .. c+R-1 of a problem that runs alone, with the plain
+        one-problem kernels (the same phases and stages as the batched
+        rounds: swept cycles through ps_swept_run, tail stages through
+        ps_standard_run), without the batched kernels' per-work-item
+        lookups."""
+        cls = t.cls
+        sw, st, msw, mst, f, fstd = cls.solo_modules()
+        if cls.solo_consts is not t:
+            cls.cpu.set_globals(*t.problem.globals, source_mod=plugins.SourceModuleFacade([msw, mst]))
+            cls.solo_consts = t
+        G, P, T = t.times.G, cls.P, 2 * cls.P + cls.its
+        c0, c1 = t.c, t.c + R
+        if G >= 0 and c0 <= G + 1:
+            plan = hip.SweptPlan(state=t.ring.ptr, nx=t.nx, ny=t.ny, b=cls.b, ops=cls.ops, its=cls.its, T=T,
+                                 threads=sw.threads, store_all=0, up=f["UpPyramid"], ybridge=f["YBridge"],
+                                 xbridge=f["XBridge"], octahedron=f["Octahedron"], down=f["DownPyramid"])
+            hip.swept_run(plan, G, c0, min(c1, G + 2))
+        lo = max(c0, G + 2 if G >= 0 else 1)
+        if c1 > lo:
+            done = G + 1
+            base = done * P if G >= 0 else 0
+            plan = hip.StandardPlan(state=t.ring.ptr, nx=t.nx, ny=t.ny, its=cls.its, T=T, bx=st.bx, by=st.by,
+                                    threads=st.threads, standard=fstd)
+            hip.standard_run(plan, base + lo - done, base + c1 - done)
+
+    def _track(self, R):
+        """Bound the queued GPU work (see ``max_inflight``) and keep the
+        measured time per round."""
+        if not self.max_inflight:
+            return
+        ev = hip.Event()
+        ev.record()
+        self.inflight.append((ev, R))
+        while len(self.inflight) > self.max_inflight:
+            done, rounds = self.inflight.pop(0)
+            done.synchronize()
+            if self._last_done is not None:
+                per = self._last_done.elapsed_ms(done) / 1e3 / rounds
+                self.round_s = per if self.round_s is None else 0.7 * self.round_s + 0.3 * per
+            self._last_done = done
+
     def step(self):
         """Admit what fits, run rounds of every class up to the next
         retirement (or admission check), retire finished problems."""
@@ -369,6 +435,18 @@ class Engine:
         if not self.running:
             if self.queue:
                 time.sleep(max(0.0, min(t.arrival for t in self.queue) - self.now()))
+            return
+        if self.solo and len(self.running) == 1:
+            # one problem: the plain kernels, as a one-at-a-time solve runs it
+            t = self.running[0]
+            R = self._rounds_ahead({t})
+            self._run_solo(t, R)
+            self.tables[t.cls.key].dirty = True  # its round counter moved on the host only
+            self.rounds += R
+            self._track(R)
+            t.c += R
+            if t.c >= t.c_end:
+                self._retire(t)
             return
         if self.policy == "short_first":
             active = short_first(self.running, self.now())
@@ -391,18 +469,7 @@ class Engine:
             else:
                 hip.multi_run(tab.plan, R)
         self.rounds += R
-        if self.max_inflight:
-            ev = hip.Event()
-            ev.record()
-            self.inflight.append((ev, R))
-            while len(self.inflight) > self.max_inflight:
-                done, rounds = self.inflight.pop(0)
-                done.synchronize()
-                # GPU time per round, from consecutive step events
-                if self._last_done is not None:
-                    per = self._last_done.elapsed_ms(done) / 1e3 / rounds
-                    self.round_s = per if self.round_s is None else 0.7 * self.round_s + 0.3 * per
-                self._last_done = done
+        self._track(R)
         for t in list(self.running):
             if t not in active:
                 continue

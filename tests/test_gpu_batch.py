@@ -164,3 +164,16 @@ def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(dev
         for tag, r in t["results"].items():
             assert np.array_equal(r, first[tag]), (t, tag)
     assert Tuner(path=path).method("heat", "fp64", 1) == rec["method"]
+
+
+@pytest.mark.parametrize("method", ["swept", "standard"])
+def test_solo_path_equals_batched_rounds(device, method):
+    """A problem running alone uses the plain kernels (swept cycles, then the
+    exact-step tail); the same problem through batched rounds, and the
+    oracle, give the same bits."""
+    p, step = _euler_problem(96, 64, 10, b=16)  # 20 stages: swept cycles and a tail
+    p.method = method
+    a, _ = _run([p])
+    b, _ = _run([p], solo=False)
+    want = oracle_stages(step, p.ic, 2, 20)
+    assert np.array_equal(a[0].result, want) and np.array_equal(b[0].result, want)
