@@ -255,3 +255,67 @@ def test_large_euler_swept_equals_standard_conserves(device):
     assert np.array_equal(sw.final_state, st.final_state)
     for v in range(4):
         assert abs(sw.final_state[v].sum() - ic[v].sum()) <= 1e-11 * np.abs(ic[v]).sum()
+
+
+# ------------------------------------------- the reference's own test sweeps
+# tests.py `testing` decorator (tests.py:54-77): block sizes 8, 12, 16, 24,
+# swept and standard; the heat and vortex tests compare with the analytic
+# solutions through numpy.allclose (tests.py:410-519, 106-163).
+
+@pytest.mark.parametrize("swept", [True, False])
+@pytest.mark.parametrize("b", [8, 12, 16, 24])
+def test_reference_sweep_heat_rk2_384(device, b, swept):
+    """testHeatRungeKuttaTwo (tests.py:466-519): heat RK2 on 384^2 over the
+    reference's block sizes, from the analytic initial condition: the
+    oracle's bits, and numpy.allclose against the analytic decay at its
+    default tolerances, as the reference asserts."""
+    n, steps = 384, 60
+    dx = 1.0 / n
+    dt = 0.1 * dx * dx
+    x = o_heat.grid(n)
+    ic = np.outer(np.sin(2 * np.pi * x), np.sin(2 * np.pi * x))[None]
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, False]
+    c = o_heat.courant_py(1.0, dt, dx)
+    step = lambda q, qb, gg: o_heat.step_py(q[0], qb[0], gg, c, False)[None]
+    sol = gpu_solve("heat", ic, g, 1, 2, b, swept=swept, exact=True)
+    assert np.array_equal(sol.final_state, oracle_stages(step, ic, 2, 2 * steps))
+    assert np.allclose(sol.final_state[0], ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt))
+
+
+@pytest.mark.parametrize("b", [12, 24])
+def test_reference_block_sizes_euler(device, b):
+    """Euler with the reference's non-power-of-two block sizes (k=2: b=12
+    gives P=2, b=24 gives P=5): swept == standard == oracle, bitwise."""
+    n, steps = 96 if b == 12 else 120, 15
+    ic, g, step = euler_case(n, steps, eps=4.0)
+    sw = gpu_solve("euler", ic, g, 2, 2, b, swept=True, exact=True)
+    want = oracle_stages(step, ic, 2, 2 * steps)
+    assert np.array_equal(sw.final_state, want)
+
+
+def test_reference_heat_forward_euler_120_b24(device):
+    """testHeatForwardEuler (tests.py:410-464): 120^2, b=24, against the
+    analytic solution with numpy.allclose's default tolerances."""
+    n = 120
+    dx = 1.0 / n
+    dt = 0.1 * dx * dx
+    steps = 200
+    x = o_heat.grid(n)
+    ic = np.outer(np.sin(2 * np.pi * x), np.sin(2 * np.pi * x))[None]
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
+    sol = gpu_solve("heat", ic, g, 1, 1, 24, swept=True, exact=True)
+    exact = ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt)
+    assert np.allclose(sol.final_state[0], exact)
+
+
+def test_reference_euler_vortex_120_b24(device):
+    """testEulerVortex (tests.py:106-163): 120^2, 200 steps, b=24: the
+    oracle's bits, and the advected vortex within the scheme's truncation
+    error (the reference prints its maximum error)."""
+    n, steps = 120, 200
+    ic, g, step = euler_case(n, steps)
+    sol = gpu_solve("euler", ic, g, 2, 2, 24, swept=True, exact=True)
+    assert np.array_equal(sol.final_state, oracle_stages(step, ic, 2, 2 * steps))
+    X, Y = np.meshgrid(o_euler.grid(n), o_euler.grid(n), indexing="ij")
+    ex = o_euler.vortex(X, Y, steps * g[2], 1.4)
+    assert np.abs(sol.final_state[0] - ex[0]).max() < 0.2
