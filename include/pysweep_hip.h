@@ -132,6 +132,9 @@ typedef struct ps_swept_plan {
                        complete after its cycle, e.g. to record history);
                        0: write only the cells a later phase reads and the
                        final s levels                                      */
+    int tile_rows;  /* tile rows from x = 0 (plus the cycle's offset) that
+                       the GPU computes; 0: all (the rest of the rows are a
+                       host's share, core/process.py:138-176)              */
     ps_function up, ybridge, xbridge, octahedron, down;
 } ps_swept_plan;
 

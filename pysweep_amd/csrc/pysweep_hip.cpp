@@ -136,7 +136,7 @@ int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xof
     g.ny = p->ny;
     g.T = p->T;
     g.xoff = xoff;
-    g.tiles_x = p->nx / p->b;
+    g.tiles_x = p->tile_rows > 0 ? p->tile_rows : p->nx / p->b;
     g.tiles_y = p->ny / p->b;
     g.keep_from = keep_from;
     void *state = p->state;
@@ -388,6 +388,7 @@ static int swept_run(const ps_swept_plan *p, int G, int c_begin, int c_end)
     if (P < 1) return fail("block size too small: b/(2k)-1 must be >= 1");
     if (p->nx % b != 0 || p->ny % b != 0) return fail("nx and ny must be multiples of the block size");
     if (p->T != 2 * P + s) return fail("ring must hold 2P+s levels");
+    if (p->tile_rows < 0 || p->tile_rows > p->nx / b) return fail("tile_rows out of range");
     if (G < 0 || c_begin < 0 || c_end > G + 2 || c_begin > c_end) return fail("cycle range out of bounds");
     const int h = b / 2, base = s - 1;
     // The final s levels (s-1+(G+1)P-s+1 ..) are always written in full.

@@ -95,7 +95,7 @@ class SweptRun(_Run):
     """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True,
-                 real="double", generic=False):
+                 real="double", generic=False, tile_rows=0):
         P = b // (2 * ops) - 1
         if b % (2 * ops) or P < 1:
             raise ValueError(f"invalid blocksize {b} for operating_points {ops}")
@@ -109,7 +109,8 @@ class SweptRun(_Run):
         self.modules = [mod]
         f = {n: mod.get_function(n) for n in kernels.SWEPT_KERNELS}
         self.plan = hip.SweptPlan(state=self.ring.ptr, nx=nx, ny=ny, b=b, ops=ops, its=its, T=self.T,
-                                  threads=self.spec.threads, up=f["UpPyramid"], ybridge=f["YBridge"],
+                                  threads=self.spec.threads, tile_rows=tile_rows, up=f["UpPyramid"],
+                                  ybridge=f["YBridge"],
                                   xbridge=f["XBridge"], octahedron=f["Octahedron"], down=f["DownPyramid"])
         self.tail = None
         if tail_standard:

@@ -36,7 +36,7 @@ class Grid(ctypes.Structure):
 class SweptPlan(ctypes.Structure):
     _fields_ = [("state", c_void_p), ("nx", c_int), ("ny", c_int), ("b", c_int), ("ops", c_int),
                 ("its", c_int), ("T", c_int), ("threads", c_int), ("store_all", c_int),
-                ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p),
+                ("tile_rows", c_int), ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p),
                 ("octahedron", c_void_p), ("down", c_void_p)]
 
 

@@ -85,6 +85,22 @@ def full_step_levels(s, lo, hi):
 PHASES = ("UpPyramid", "YBridge", "XBridge", "Octahedron", "DownPyramid")
 
 
+def cycle_phases(c, G, P, s, b):
+    """Phases of swept cycle c (0 FirstPrism, 1..G UpPrism, G+1 LastPrism;
+    core/functions.py:12-88) as (kind, A0, gts0, xoff, yoff): tile-local
+    level r reads absolute level A0+r-1 and writes A0+r with global stage
+    gts0+r-1; tiles start at rows xoff + I*b and columns yoff + J*b."""
+    h, base = b // 2, s - 1
+    xoff = (c % 2) * h
+    if c == 0:
+        return [("UpPyramid", base, 1, 0, h), ("YBridge", base, 1, 0, 0)]
+    if c <= G:
+        A, g = base + (c - 1) * P, (c - 1) * P + 1
+        return [("XBridge", A, g, xoff, 0), ("Octahedron", A, g, xoff, h), ("YBridge", A + P, g + P, xoff, 0)]
+    A, g = base + G * P, G * P + 1
+    return [("XBridge", A, g, xoff, 0), ("DownPyramid", A, g, xoff, h)]
+
+
 def phase_set(kind, r, b, k, P):
     """Active rectangle (x0, x1, y0, y1) of tile-local level r of a phase
     (core/block.py:123-164; core/pysweep.cu:64-249)."""

@@ -9,8 +9,10 @@ Same construction and attribute API as the reference:
 Attributes (core/io.py:35-75): simulation (True = swept), verbose,
 intermediate, operating, blocksize (b, b, 1), share, exid, globals
 [t0, tf, dt, dx, dy, *equation args], cpu / gpu plugin paths, output,
-dtypeStr / dtype.  The MPI split (core/process.py) collapses to one rank
-with the whole grid on the GPU, so ``share`` must be 1.
+dtypeStr / dtype.  The MPI ranks (core/process.py) collapse to one process:
+with share = 1 the whole grid is on the GPU; with share < 1 (swept only) the
+GPU takes the first ceil(share * nx/b) block rows and the host the rest,
+computed with the plugin's Python step (pysweep_amd.share).
 
 Additions: ``exact`` (take exactly N steps instead of the reference's
 quantization to multiples of the pyramid height), ``history`` (keep every
@@ -145,9 +147,15 @@ class Solver(object):
 
     def sweptSolve(self):
         """FirstPrism, G UpPrism cycles, LastPrism (core/solver.py:158-176)."""
-        from pysweep_amd import hip
         run = self._run
         collect = self._collector()
+        if self.share < 1:
+            t0 = time.perf_counter()
+            self.final_level = run.run(self.times, on_level=collect)
+            self.solve_seconds = time.perf_counter() - t0
+            self.globalTimeStep = self.final_level - (self.intermediate - 1) + 1
+            return
+        from pysweep_amd import hip
         start, stop = hip.Event(), hip.Event()
         start.record()
         self.final_level = run.run(self.times, on_level=collect)
@@ -178,9 +186,11 @@ class Solver(object):
             raise ValueError("no initial conditions")
         if len(self.initialConditions.shape) != 3:
             raise ValueError("initial conditions must be an array (nv, nx, ny)")
-        if self.share != 1 and self.share != 1.0:
-            raise NotImplementedError(
-                "this build runs the whole grid on one GPU (share=1); the CPU/GPU split is not supported")
+        if not 0 <= self.share <= 1:
+            raise ValueError("share must be in [0, 1]")
+        if self.share < 1 and not self.simulation:
+            raise NotImplementedError("the CPU/GPU share split runs the swept decomposition; "
+                                      "use share=1 for the standard one")
         if self.dtype != numpy.float64:
             raise NotImplementedError("device state is float64 (set dtype: float64)")
         b = self.blocksize[0]
@@ -196,10 +206,24 @@ class Solver(object):
             raise ValueError("cpu and gpu plugin files must be set")
 
     def _setup_device(self):
-        from pysweep_amd import device, hip
-        hip.ensure_init(0)
         nv, nx, ny = self.initialConditions.shape
         t = dict(self.tuning)
+        if self.simulation and self.share < 1:
+            # CPU/GPU split (core/process.py:81-176): block rows divided by share
+            from pysweep_amd import share
+            if self.share > 0:
+                from pysweep_amd import hip
+                hip.ensure_init(0)
+            self._run = share.SharedSweptRun(self.cpu, self.gpu, nv, self.operating, self.intermediate, nx, ny,
+                                             self.blocksize[0], self.share, slots=t.get("slots"),
+                                             threads=t.get("threads"), generic=t.get("generic", False))
+            self.share = self._run.share  # effective share (process.py:148)
+            self.cpu.set_globals(*self.globals)
+            self._run.set_globals(self.cpu, self.globals)
+            self._run.upload_ic(self.initialConditions)
+            return
+        from pysweep_amd import device, hip
+        hip.ensure_init(0)
         if self.simulation:
             self._run = device.SweptRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                         self.blocksize[0], slots=t.get("slots"), threads=t.get("threads"),

@@ -175,14 +175,15 @@ def test_yaml_rejects_bad_blocksize(tmp_path):
         Solver(np.zeros((1, 8, 8)), str(y))
 
 
-def test_solver_rejects_split_share():
+def test_solver_rejects_out_of_range_share():
     from pysweep_amd import Solver
     s = Solver(np.zeros((1, 16, 16)), sendWarning=False)
-    s.share = 0.5
+    s.share = 1.5
+    s.blocksize = (8, 8, 1)
     s.globals = [0, 1, 0.1, 0.1, 0.1, True]
     s.setCPU(eq_path("example.py"))
     s.setGPU(eq_path("example.cu"))
-    with pytest.raises(NotImplementedError, match="share"):
+    with pytest.raises(ValueError, match="share"):
         s()
 
 
