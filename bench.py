@@ -274,6 +274,39 @@ def large_instances():
     return out
 
 
+def heat_cpu_case():
+    """The reference's CPU-runnable case: heat 384^2, b=8, forward Euler,
+    100 steps (99 stages with the reference's quantization), swept, from
+    sin(2 pi x) sin(2 pi y) plus seeded noise: on the GPU alone (share 1),
+    on the host alone (share 0: the plugin's Python step over each level's
+    cells, one core), and split (share 0.5); all three bitwise equal."""
+    import numpy as np
+
+    from pysweep_amd import Solver
+    from pysweep_amd.equations import path as eq_path
+    from pysweep_amd import workloads
+    n = 384
+    dx = 1.0 / n
+    dt = 0.1 * dx * dx
+    rng = np.random.default_rng(0)
+    ic = workloads.heat_mode(n, n) + 1e-3 * rng.uniform(-1, 1, (1, n, n))
+    out = {"workload": "heat-384x384-b8-100steps-fe-swept"}
+    finals = {}
+    for share in (1.0, 0.5, 0.0):
+        sol = Solver(ic, sendWarning=False)
+        sol.simulation, sol.operating, sol.intermediate, sol.blocksize = True, 1, 1, (8, 8, 1)
+        sol.share = share
+        sol.globals = [0.0, 100 * dt, dt, dx, dx, 1.0, True]
+        sol.setCPU(eq_path("heat.py"))
+        sol.setGPU(eq_path("heat.cu"))
+        sol.loadCPUModule()
+        sol()
+        finals[share] = sol.final_state
+        out[f"share_{share:g}"] = {"gcups": sol.gcups, "seconds": sol.solve_seconds, "stages": sol.stages}
+    out["bitwise_equal"] = bool(np.array_equal(finals[1.0], finals[0.0]) and np.array_equal(finals[1.0], finals[0.5]))
+    return out
+
+
 def numpy_equal(a, b):
     import numpy as np
     return bool(np.array_equal(a, b))
@@ -398,6 +431,8 @@ def main():
     st.close()
     if a.large:
         out["large_instances"] = large_instances()
+    if not a.no_cpu:
+        out["heat_cpu_case"] = heat_cpu_case()
     if a.batch:
         out["batched"] = batched_workload(a.batch)
     if a.serve:

@@ -10,9 +10,10 @@ float32 dataset and reads it back; here it is built in memory and rounded to
 float32 the same way.  Each run appends a record to ``log.yaml`` (as
 core/io.py:10-22 does), with the device solve time and GCUPS added.
 
-Differences: ``--share`` must be 1 (the whole grid runs on the GPU), the
-figure options (-p) are not provided, and ``--exact`` takes exactly nt steps
-instead of the reference's quantization to whole swept cycles.
+Differences: ``--share`` defaults to 1 (the whole grid on the GPU; below 1
+the host takes the remaining block rows, swept only), the figure options
+(-p) are not provided, and ``--exact`` takes exactly nt steps instead of
+the reference's quantization to whole swept cycles.
 """
 import argparse
 import math
@@ -137,7 +138,8 @@ def parser():
                    help="block size b, a multiple of 2k (k points on each side of the operating point)")
     p.add_argument("-nx", "--spacesteps", default=24, nargs="?", type=int, help="points in x and y")
     p.add_argument("-nt", "--timesteps", default=10, nargs="?", type=int, help="time steps")
-    p.add_argument("-s", "--share", default=1.0, nargs="?", type=float, help="GPU share (must be 1)")
+    p.add_argument("-s", "--share", default=1.0, nargs="?", type=float,
+                   help="GPU share of the block rows, 0..1 (below 1 the host takes the rest; swept only)")
     p.add_argument("--clean", action="store_true", help="accepted for compatibility; no files are written")
     p.add_argument("--ignore", action="store_true", help="ignore warnings")
     p.add_argument("-v", "--verbose", action="store_true")
