@@ -155,7 +155,7 @@ def command_line(argv=None):
         return None
     solver = FUNCTIONS[args.function](args)
     print(f"{'swept' if solver.simulation else 'standard'} {args.function} {args.spacesteps}^2 "
-          f"b={args.blocksize}: {solver.stages} stages in {solver.solve_seconds:.4f} s on the GPU, "
+          f"b={args.blocksize} share={solver.share:g}: {solver.stages} stages in {solver.solve_seconds:.4f} s, "
           f"{solver.gcups:.3f} GCUPS")
     return solver
 
