@@ -58,6 +58,7 @@ class ModuleSpec:
     multi: bool = False   # instance-table kernels with per-instance constants
     mpss: int = 1         # pyramid height (standard modules of a multi class)
     generic: bool = False  # force the per-cell step even if the plugin has a level update
+    pad: int = 0          # swept: LDS row padding (SY = b + pad)
 
     @property
     def real_bytes(self):
@@ -68,7 +69,7 @@ class ModuleSpec:
         name = os.path.basename(self.user_path)
         if self.kind == "swept":
             return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads,
-                                         real=self.real, multi=self.multi, generic=self.generic)
+                                         real=self.real, multi=self.multi, generic=self.generic, pad=self.pad)
         if self.kind == "standard":
             return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
                                             mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic)
@@ -79,15 +80,17 @@ class ModuleSpec:
 
     def lds_bytes(self):
         if self.kind == "swept":
-            tile = self.slots * self.nv * self.b * self.b * self.real_bytes
-            return tile + level_scratch_bytes(self.user_path, self.generic, self.b, self.b, swept_faces(self.b, self.ops))
+            tile = self.slots * self.nv * self.b * (self.b + self.pad) * self.real_bytes
+            return tile + level_scratch_bytes(self.user_path, self.generic, self.b, self.b + self.pad,
+                                              swept_faces(self.b, self.ops))
         sx, sy = self.bx + 2 * self.ops, self.by + 2 * self.ops
         tile = (self.its + 1) * self.nv * sx * sy * self.real_bytes
         return tile + level_scratch_bytes(self.user_path, self.generic, sx, sy,
                                           max((self.bx + 1) * self.by, self.bx * (self.by + 1)), "standard")
 
     def describe(self):
-        g = (f"b={self.b} slots={self.slots}" if self.kind == "swept" else f"tile={self.bx}x{self.by}")
+        g = (f"b={self.b} slots={self.slots}" + (f" pad={self.pad}" if self.pad else "")
+             if self.kind == "swept" else f"tile={self.bx}x{self.by}")
         return (f"{self.kind}{'-multi' if self.multi else ''}{'-generic' if self.generic else ''} "
                 f"{os.path.basename(self.user_path)} {self.real} "
                 f"nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}")
@@ -149,16 +152,16 @@ def default_slots(nv, its, b, P, real_bytes=8, scratch=0):
 
 
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
-               multi=False, generic=False):
+               multi=False, generic=False, pad=0):
     if b % (2 * ops) != 0:
         raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
     P = b // (2 * ops) - 1
     if P < 1:
         raise ValueError(f"blocksize {b} too small for operating_points={ops} (needs b >= {4 * ops})")
     rb = 8 if real == "double" else 4
-    level = nv * b * b * rb
+    level = nv * b * (b + pad) * rb
     full = 2 * P + its
-    scratch = level_scratch_bytes(user_path, generic, b, b, swept_faces(b, ops))
+    scratch = level_scratch_bytes(user_path, generic, b, b + pad, swept_faces(b, ops))
     if slots is None:
         slots = default_slots(nv, its, b, P, rb, scratch)
     slots = min(slots, full)
@@ -172,7 +175,7 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
         if level > 16 * 1024:
             threads = 512
     return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads,
-                      real=real, multi=multi, generic=generic)
+                      real=real, multi=multi, generic=generic, pad=pad)
 
 
 def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scratch=None, threads=256):

@@ -319,3 +319,22 @@ def test_reference_euler_vortex_120_b24(device):
     X, Y = np.meshgrid(o_euler.grid(n), o_euler.grid(n), indexing="ij")
     ex = o_euler.vortex(X, Y, steps * g[2], 1.4)
     assert np.abs(sol.final_state[0] - ex[0]).max() < 0.2
+
+
+@pytest.mark.parametrize("pad", [1, 2])
+def test_padded_lds_rows(device, pad):
+    """Swept tiles whose LDS rows are padded (SY = b + pad) give the same bits."""
+    from pysweep_amd import device as dev, hip, plugins, schedule
+    from pysweep_amd.equations import path as eq_path
+    n, steps = 64, 7
+    ic, g, step = euler_case(n, steps, eps=4.0)
+    cpu = plugins.load_cpu_module(eq_path("euler.py"))
+    cpu.set_globals(*g)
+    run = dev.SweptRun(eq_path("euler.cu"), 4, 2, 2, n, n, 16, pad=pad)
+    run.set_globals(cpu, g)
+    run.upload_ic(ic)
+    times = schedule.swept_times(steps, 16, 2, 2, exact=True)
+    last = run.run(times)
+    hip.synchronize()
+    assert np.array_equal(run.level(last), oracle_stages(step, ic, 2, 2 * steps))
+    run.close()
