@@ -41,17 +41,69 @@ __device__ __forceinline__ eu_q eu_at(const R *s, int i)
     return q;
 }
 
-/* gM1*(E - (m*m + n*n)/(2*rho))  (euler.py:284-293) */
+/*
+ * Division.  n/d is formed as the gfx950 compiler's IEEE double division
+ * forms it -- reciprocal estimate, two Newton-Raphson refinements, product,
+ * one residual correction, special-case fix-up -- less its operand-scaling
+ * steps, which leave every value unchanged unless an operand lies within
+ * 2^53 of the subnormal range, d is subnormal or the exponents differ by 768
+ * or more (no Euler state comes near).  The quotient is therefore bitwise
+ * the one `n / d` gives.  The refined reciprocal depends on d alone, so the
+ * quotients of one face that share a denominator share it (6 reciprocals
+ * serve the 19 quotients of a face).
+ */
+struct eu_rcp {
+    double d, y; /* denominator, refined reciprocal */
+};
+
+__device__ __forceinline__ eu_rcp eu_recip(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    double e = __builtin_fma(-d, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-d, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    eu_rcp q;
+    q.d = d;
+    q.y = y;
+    return q;
+}
+
+/* the reciprocal of 2d from that of d: doubling d halves every step of
+ * eu_recip exactly */
+__device__ __forceinline__ eu_rcp eu_twice(const eu_rcp &q)
+{
+    eu_rcp h;
+    h.d = 2 * q.d;
+    h.y = q.y * 0.5;
+    return h;
+}
+
+__device__ __forceinline__ double eu_quo(double n, const eu_rcp &q)
+{
+    double z = n * q.y;
+    const double rem = __builtin_fma(-q.d, z, n);
+    z = __builtin_fma(rem, q.y, z);
+    return __builtin_amdgcn_div_fixup(z, q.d, n);
+}
+
+/* gM1*(E - (m*m + n*n)/(2*rho))  (euler.py:284-293); r2 is the reciprocal
+ * of 2*rho */
+__device__ __forceinline__ double eu_pressure(const eu_q &q, const eu_rcp &r2)
+{
+    return GAM_M1 * (q.e - eu_quo(q.m * q.m + q.n * q.n, r2));
+}
+
 __device__ __forceinline__ double eu_pressure(const eu_q &q)
 {
-    return GAM_M1 * (q.e - (q.m * q.m + q.n * q.n) / (2 * q.r));
+    return eu_pressure(q, eu_recip(2 * q.r));
 }
 
 /* q1 + min(Pr,1)/2*(q2-q1), Pr = num/den zeroed when nan, inf, >1e6 or <0
  * (euler.py:250-256) */
 __device__ __forceinline__ eu_q eu_limit(const eu_q &q1, const eu_q &q2, double num, double den)
 {
-    double pr = num / den;
+    double pr = eu_quo(num, eu_recip(den));
     if (isnan(pr) || isinf(pr) || pr > 1e6 || pr < 0) pr = 0.0;
     const double h = (1.0 < pr ? 1.0 : pr) / 2;
     eu_q o;
@@ -62,22 +114,23 @@ __device__ __forceinline__ eu_q eu_limit(const eu_q &q1, const eu_q &q2, double 
     return o;
 }
 
-/* central flux of a face, x direction (euler.py:181-197) or y (225-241) */
+/* central flux of a face, x direction (euler.py:181-197) or y (225-241);
+ * rl, rr are the reciprocals of L.r and R.r */
 template <bool XDIR>
-__device__ __forceinline__ eu_q eu_flux(const eu_q &L, const eu_q &R)
+__device__ __forceinline__ eu_q eu_flux(const eu_q &L, const eu_q &R, const eu_rcp &rl, const eu_rcp &rr)
 {
-    const double pl = eu_pressure(L), pr = eu_pressure(R);
+    const double pl = eu_pressure(L, eu_twice(rl)), pr = eu_pressure(R, eu_twice(rr));
     eu_q f;
     if (XDIR) {
         f.r = L.m + R.m;
-        f.m = L.m * L.m / L.r + pl + R.m * R.m / R.r + pr;
-        f.n = L.m * L.n / L.r + R.m * R.n / R.r;
-        f.e = (L.e + pl) * L.m / L.r + (R.e + pr) * R.m / R.r;
+        f.m = eu_quo(L.m * L.m, rl) + pl + eu_quo(R.m * R.m, rr) + pr;
+        f.n = eu_quo(L.m * L.n, rl) + eu_quo(R.m * R.n, rr);
+        f.e = eu_quo((L.e + pl) * L.m, rl) + eu_quo((R.e + pr) * R.m, rr);
     } else {
         f.r = L.n + R.n;
-        f.m = L.m * L.n / L.r + R.m * R.n / R.r;
-        f.n = L.n * L.n / L.r + pl + R.n * R.n / R.r + pr;
-        f.e = (L.e + pl) * L.n / L.r + (R.e + pr) * R.n / R.r;
+        f.m = eu_quo(L.m * L.n, rl) + eu_quo(R.m * R.n, rr);
+        f.n = eu_quo(L.n * L.n, rl) + pl + eu_quo(R.n * R.n, rr) + pr;
+        f.e = eu_quo((L.e + pl) * L.n, rl) + eu_quo((R.e + pr) * R.n, rr);
     }
     return f;
 }
@@ -85,20 +138,21 @@ __device__ __forceinline__ eu_q eu_flux(const eu_q &L, const eu_q &R)
 /* spectral radius of the Roe average times (L - R); radius zeroed when nan
  * (euler.py:258-282) */
 template <bool XDIR>
-__device__ __forceinline__ eu_q eu_dissipation(const eu_q &L, const eu_q &R)
+__device__ __forceinline__ eu_q eu_dissipation(const eu_q &L, const eu_q &R, const eu_rcp &rl, const eu_rcp &rr)
 {
     const double sl = sqrt(L.r), sr = sqrt(R.r);
-    const double den = 1 / (sl + sr);
+    const double den = eu_quo(1.0, eu_recip(sl + sr));
     const double a0 = sl * sr;
-    const double a1 = (sl * (L.m / L.r) + sr * (R.m / R.r)) * den;
-    const double a2 = (sl * (L.n / L.r) + sr * (R.n / R.r)) * den;
-    const double a3 = (sl * (L.e / L.r) + sr * (R.e / R.r)) * den;
+    const double a1 = (sl * eu_quo(L.m, rl) + sr * eu_quo(R.m, rr)) * den;
+    const double a2 = (sl * eu_quo(L.n, rl) + sr * eu_quo(R.n, rr)) * den;
+    const double a3 = (sl * eu_quo(L.e, rl) + sr * eu_quo(R.e, rr)) * den;
     eu_q roe;
     roe.r = a0;
     roe.m = a0 * a1;
     roe.n = a0 * a2;
     roe.e = a0 * a3;
-    double rad = sqrt(GAMMA * eu_pressure(roe) / a0) + fabs(XDIR ? a1 : a2);
+    const eu_rcp ra = eu_recip(a0);
+    double rad = sqrt(eu_quo(GAMMA * eu_pressure(roe, eu_twice(ra)), ra)) + fabs(XDIR ? a1 : a2);
     if (isnan(rad)) rad = 0.0;
     eu_q d;
     d.r = rad * (L.r - R.r);
@@ -122,16 +176,19 @@ __device__ __forceinline__ eu_q eu_divergence(const S *s, int idx, int stride)
 
     eu_q L = eu_limit(w, c, pc - pw, pw - pww);
     eu_q R = eu_limit(c, w, pc - pw, pe - pc);
-    eu_q f = eu_flux<XDIR>(L, R);
+    eu_rcp rl = eu_recip(L.r), rr = eu_recip(R.r);
+    eu_q f = eu_flux<XDIR>(L, R, rl, rr);
     acc.r = acc.r + f.r; acc.m = acc.m + f.m; acc.n = acc.n + f.n; acc.e = acc.e + f.e;
-    f = eu_dissipation<XDIR>(L, R);
+    f = eu_dissipation<XDIR>(L, R, rl, rr);
     acc.r = acc.r + f.r; acc.m = acc.m + f.m; acc.n = acc.n + f.n; acc.e = acc.e + f.e;
 
     L = eu_limit(c, e, pe - pc, pc - pw);
     R = eu_limit(e, c, pe - pc, pee - pe);
-    f = eu_flux<XDIR>(L, R);
+    rl = eu_recip(L.r);
+    rr = eu_recip(R.r);
+    f = eu_flux<XDIR>(L, R, rl, rr);
     acc.r = acc.r - f.r; acc.m = acc.m - f.m; acc.n = acc.n - f.n; acc.e = acc.e - f.e;
-    f = eu_dissipation<XDIR>(L, R);
+    f = eu_dissipation<XDIR>(L, R, rl, rr);
     acc.r = acc.r - f.r; acc.m = acc.m - f.m; acc.n = acc.n - f.n; acc.e = acc.e - f.e;
 
     acc.r = acc.r * 0.5; acc.m = acc.m * 0.5; acc.n = acc.n * 0.5; acc.e = acc.e * 0.5;
@@ -241,7 +298,8 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
             const double pc = P[x * SY + y], pe = P[(x + 1) * SY + y];
             const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
             const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
-            const eu_q f = eu_flux<true>(L, Rr), d = eu_dissipation<true>(L, Rr);
+            const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
+            const eu_q f = eu_flux<true>(L, Rr, rl, rr), d = eu_dissipation<true>(L, Rr, rl, rr);
             double *o = F + t;
             o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
             o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
@@ -282,7 +340,8 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
             const double pc = P[x * SY + y], pe = P[x * SY + y + 1];
             const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
             const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
-            const eu_q f = eu_flux<false>(L, Rr), d = eu_dissipation<false>(L, Rr);
+            const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
+            const eu_q f = eu_flux<false>(L, Rr, rl, rr), d = eu_dissipation<false>(L, Rr, rl, rr);
             double *o = F + t;
             o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
             o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
