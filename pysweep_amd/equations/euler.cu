@@ -237,8 +237,9 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  *
  *   cur    level r-1 in LDS, variable v of cell (x, y) at
  *          cur[v*VARS + x*SY + y]; prev is level r-2 (the base of a final
- *          stage) and out receives level r, with the same strides (three
- *          separate slots, so the caller may keep its window circular);
+ *          stage) and out receives level r, with the same strides (slots
+ *          passed separately, so the caller may keep its window circular;
+ *          out may equal prev);
  *   sc     scratch of PS_LEVEL_SCRATCH(SX * SY, PS_MAXFACES) doubles,
  *          PS_MAXFACES being the most faces of one direction of any
  *          rectangle the harness passes ((x1-x0+1)*(y1-y0) or
@@ -248,6 +249,9 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  */
 #define PS_HAS_LEVEL 1
 #define PS_LEVEL_SCRATCH(CELLS, FACES) ((CELLS) + 8 * (FACES))
+/* prev is read only at the cell being written, by the lane writing it, so
+ * out may be prev's own slot (a swept window of ITS levels suffices) */
+#define PS_LEVEL_INPLACE 1
 
 /* t / w for 0 <= t < 2^22, w >= 1, rw = 1.0f / w: the float quotient is
  * within one of the true one; one correction makes it exact */

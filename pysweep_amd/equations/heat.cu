@@ -57,6 +57,8 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  */
 #define PS_HAS_LEVEL 1
 #define PS_LEVEL_SCRATCH(CELLS, FACES) 1
+/* prev is read only at the cell being written, so out may be prev's slot */
+#define PS_LEVEL_INPLACE 1
 /* the standard kernel keeps the per-cell step (one pass over the tile,
  * measured faster there) */
 #define PS_LEVEL_SWEPT_ONLY 1

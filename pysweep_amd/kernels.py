@@ -132,6 +132,22 @@ def level_update_in_standard(user_path):
                      re.M) is None
 
 
+def level_in_place(user_path, generic=False):
+    """True when the plugin's level update may write level r over level r-2
+    (PS_LEVEL_INPLACE: it reads ``prev`` only at the cell it writes, before
+    writing it), so a swept window of ITS levels suffices."""
+    if generic or not has_level_update(user_path):
+        return False
+    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_LEVEL_INPLACE\b", read_plugin_source(user_path),
+                     re.M) is not None
+
+
+def min_slots(user_path, its, generic=False):
+    """Fewest LDS level slots a swept window may have: ITS+1, or ITS (at
+    least 2) when the level update works in place."""
+    return max(2, its) if level_in_place(user_path, generic) else its + 1
+
+
 def level_scratch_bytes(user_path, generic, sx, sy, faces, kind="swept"):
     """LDS bytes of a plugin's level update for an sx x sy tile."""
     f = None if generic or (kind == "standard" and not level_update_in_standard(user_path)) \
@@ -139,16 +155,18 @@ def level_scratch_bytes(user_path, generic, sx, sy, faces, kind="swept"):
     return 0 if f is None else 8 * f(sx * sy, faces)
 
 
-def default_slots(nv, its, b, P, real_bytes=8, scratch=0):
+def default_slots(nv, its, b, P, real_bytes=8, scratch=0, least=None):
     """LDS window for a swept tile: the largest within ~28 KB including the
-    level scratch, at least ITS+1 levels.  Smaller windows slide more often
-    but let more tiles share a CU, which hides the HBM latency of each
-    level's face loads.  Measured on MI355X: Euler 1024^2 b=16 with the level
-    update (12 KB scratch) 3 levels 4.7 GCUPS, 4 levels 3.9; heat 8192^2
-    b=32 2 levels 175, 3 levels 180, 5 levels 163; b=64 2 levels 245."""
+    level scratch, at least ``least`` (default ITS+1) levels.  Smaller
+    windows slide more often but let more tiles share a CU, which hides the
+    HBM latency of each level's face loads.  Measured on MI355X: Euler
+    1024^2 b=16 with the level update (12 KB scratch) 3 levels 4.7 GCUPS,
+    4 levels 3.9; heat 8192^2 b=32 2 levels 175, 3 levels 180, 5 levels 163;
+    b=64 2 levels 245."""
     level = nv * b * b * real_bytes
     full = 2 * P + its
-    return max(its + 1, min(full, (28 * 1024 - scratch) // level))
+    least = its + 1 if least is None else least
+    return max(least, min(full, (28 * 1024 - scratch) // level))
 
 
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
@@ -162,11 +180,13 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
     level = nv * b * (b + pad) * rb
     full = 2 * P + its
     scratch = level_scratch_bytes(user_path, generic, b, b + pad, swept_faces(b, ops))
+    least = min_slots(user_path, its, generic)
     if slots is None:
-        slots = default_slots(nv, its, b, P, rb, scratch)
+        slots = default_slots(nv, its, b, P, rb, scratch, least)
     slots = min(slots, full)
-    if slots < its + 1:
-        raise ValueError("an LDS window needs at least intermediate_steps+1 levels")
+    if slots < least:
+        raise ValueError(f"an LDS window needs at least {least} levels here "
+                         "(intermediate_steps+1, or intermediate_steps for an in-place level update)")
     if slots * level + scratch > LDS_MAX:
         raise ValueError(f"swept tile b={b} with {slots} levels needs {slots * level + scratch} B of LDS "
                          f"(> {LDS_MAX})")

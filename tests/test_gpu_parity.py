@@ -338,3 +338,31 @@ def test_padded_lds_rows(device, pad):
     hip.synchronize()
     assert np.array_equal(run.level(last), oracle_stages(step, ic, 2, 2 * steps))
     run.close()
+
+
+@pytest.mark.parametrize("name,b,slots", [("euler", 16, 2), ("euler", 16, 3), ("euler", 32, 2), ("euler", 32, 3),
+                                          ("heat", 16, 2), ("heat", 32, 2), ("heat", 32, 4)])
+def test_in_place_window(device, name, b, slots):
+    """A window of ITS levels (level r written over level r-2 in LDS, for
+    level updates that work in place) gives the same bits as larger windows
+    and the oracle, with RK2 (two intermediate steps)."""
+    from pysweep_amd import device as dev, hip, plugins, schedule
+    from pysweep_amd.equations import path as eq_path
+    n, steps = 64, 9
+    if name == "euler":
+        ic, g, step = euler_case(n, steps, eps=4.0)
+        nv, k = 4, 2
+    else:
+        ic, g, step = heat_case(n, False, steps, seed=3)
+        nv, k = 1, 1
+    cpu = plugins.load_cpu_module(eq_path(name + ".py"))
+    cpu.set_globals(*g)
+    run = dev.SweptRun(eq_path(name + ".cu"), nv, k, 2, n, n, b, slots=slots)
+    assert run.spec.slots == slots
+    run.set_globals(cpu, g)
+    run.upload_ic(ic)
+    times = schedule.swept_times(steps, b, k, 2, exact=True)
+    last = run.run(times)
+    hip.synchronize()
+    assert np.array_equal(run.level(last), oracle_stages(step, ic, 2, 2 * steps))
+    run.close()
