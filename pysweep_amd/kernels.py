@@ -84,7 +84,7 @@ class ModuleSpec:
             return tile + level_scratch_bytes(self.user_path, self.generic, self.b, self.b + self.pad,
                                               swept_faces(self.b, self.ops))
         sx, sy = self.bx + 2 * self.ops, self.by + 2 * self.ops
-        tile = (self.its + 1) * self.nv * sx * sy * self.real_bytes
+        tile = standard_slots(self.user_path, self.its, self.generic) * self.nv * sx * sy * self.real_bytes
         return tile + level_scratch_bytes(self.user_path, self.generic, sx, sy,
                                           max((self.bx + 1) * self.by, self.bx * (self.by + 1)), "standard")
 
@@ -148,6 +148,13 @@ def min_slots(user_path, its, generic=False):
     return max(2, its) if level_in_place(user_path, generic) else its + 1
 
 
+def standard_slots(user_path, its, generic=False):
+    """LDS level slots of the standard kernel: the ITS input levels and the
+    output, which an in-place level update writes over level A-1 (ITS >= 2)."""
+    inplace = level_update_in_standard(user_path) and level_in_place(user_path, generic) and its >= 2
+    return its if inplace else its + 1
+
+
 def level_scratch_bytes(user_path, generic, sx, sy, faces, kind="swept"):
     """LDS bytes of a plugin's level update for an sx x sy tile."""
     f = None if generic or (kind == "standard" and not level_update_in_standard(user_path)) \
@@ -198,7 +205,8 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
                       real=real, multi=multi, generic=generic, pad=pad)
 
 
-def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scratch=None, threads=256):
+def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scratch=None, threads=256,
+                  slots=None):
     """Tile (bx rows, by columns) for the standard kernel: most cells within
     the LDS budget, then rows of at least 32 doubles (coalesced wavefront
     loads), then the least halo re-reading.  ``scratch`` is the plugin's
@@ -211,13 +219,14 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scr
     the first.  Measured on MI355X (Euler 1024^2, 256 lanes): 8x16 7.1
     GCUPS, 8x32 6.6, 16x16 6.5."""
     faces_shared = scratch is not None and scratch(0, 1) > scratch(0, 0)
+    slots = its + 1 if slots is None else slots
     best = None
     for by in (64, 32, 16, 8):
         for bx in (32, 16, 8, 4, 2):
             if bx > nx or by > ny:
                 continue
             sx, sy = bx + 2 * ops, by + 2 * ops
-            lds = (its + 1) * nv * sx * sy * real_bytes
+            lds = slots * nv * sx * sy * real_bytes
             if scratch is not None:
                 lds += 8 * scratch(sx * sy, max((bx + 1) * by, bx * (by + 1)))
             if faces_shared:
@@ -239,7 +248,8 @@ def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256
                   real="double", multi=False, mpss=1, generic=False):
     if bx is None or by is None:
         sc = None if generic or not level_update_in_standard(user_path) else level_scratch(user_path)
-        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, sc, threads)
+        bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, sc, threads,
+                               standard_slots(user_path, its, generic))
     if bx > nx or by > ny:
         raise ValueError("standard tile larger than the grid")
     return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads,
