@@ -87,6 +87,24 @@ __device__ __forceinline__ double eu_quo(double n, const eu_rcp &q)
     return __builtin_amdgcn_div_fixup(z, q.d, n);
 }
 
+/* sqrt(x) as the compiler's IEEE double square root forms it -- reciprocal
+ * square-root estimate, Goldschmidt refinement, two residual corrections,
+ * x itself for +-0 and +inf -- less the scaling of x < 2^-767, which changes
+ * nothing for larger x (densities and Roe sound speeds never come near) */
+__device__ __forceinline__ double eu_sqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    h = __builtin_fma(h, r, h);
+    g = __builtin_fma(g, r, g);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    return (x == 0.0 || x == __builtin_inf()) ? x : g;
+}
+
 /* gM1*(E - (m*m + n*n)/(2*rho))  (euler.py:284-293); r2 is the reciprocal
  * of 2*rho */
 __device__ __forceinline__ double eu_pressure(const eu_q &q, const eu_rcp &r2)
@@ -140,7 +158,7 @@ __device__ __forceinline__ eu_q eu_flux(const eu_q &L, const eu_q &R, const eu_r
 template <bool XDIR>
 __device__ __forceinline__ eu_q eu_dissipation(const eu_q &L, const eu_q &R, const eu_rcp &rl, const eu_rcp &rr)
 {
-    const double sl = sqrt(L.r), sr = sqrt(R.r);
+    const double sl = eu_sqrt(L.r), sr = eu_sqrt(R.r);
     const double den = eu_quo(1.0, eu_recip(sl + sr));
     const double a0 = sl * sr;
     const double a1 = (sl * eu_quo(L.m, rl) + sr * eu_quo(R.m, rr)) * den;
@@ -152,7 +170,7 @@ __device__ __forceinline__ eu_q eu_dissipation(const eu_q &L, const eu_q &R, con
     roe.n = a0 * a2;
     roe.e = a0 * a3;
     const eu_rcp ra = eu_recip(a0);
-    double rad = sqrt(eu_quo(GAMMA * eu_pressure(roe, eu_twice(ra)), ra)) + fabs(XDIR ? a1 : a2);
+    double rad = eu_sqrt(eu_quo(GAMMA * eu_pressure(roe, eu_twice(ra)), ra)) + fabs(XDIR ? a1 : a2);
     if (isnan(rad)) rad = 0.0;
     eu_q d;
     d.r = rad * (L.r - R.r);

@@ -1,4 +1,5 @@
-"""The Euler plugin's division helpers against the GPU's own IEEE division.
+"""The Euler plugin's division and square-root helpers against the GPU's
+own IEEE division and square root.
 
 euler.cu forms n/d from a shared, Newton-refined reciprocal of d
 (eu_recip / eu_quo) instead of the compiler's full division sequence, and
@@ -6,8 +7,9 @@ the reciprocal of 2d by halving that of d (eu_twice).  The claim is that
 the quotients are bitwise those of `n / d` and `n / (2*d)` whenever the
 operands are normal, not within 2^53 of the subnormal range and less than
 2^768 apart, and that zeros, infinities and NaNs come out as IEEE division
-gives them.  The kernel below evaluates both forms side by side, compiled
-with the plugin's own source and options.
+gives them.  Likewise eu_sqrt is bitwise `sqrt` for x >= 2^-767 and for
+zeros, infinities, NaNs and negative x.  The kernels below evaluate both
+forms side by side, compiled with the plugin's own source and options.
 """
 import numpy as np
 import pytest
@@ -26,6 +28,15 @@ extern "C" __global__ void ps_div_check(const double *num, const double *den, do
     out[4 * i + 1] = eu_quo(a, r);
     out[4 * i + 2] = a / (2 * b);
     out[4 * i + 3] = eu_quo(a, eu_twice(r));
+}
+
+extern "C" __global__ void ps_sqrt_check(const double *x, double *out, int cnt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cnt)
+        return;
+    out[2 * i + 0] = sqrt(x[i]);
+    out[2 * i + 1] = eu_sqrt(x[i]);
 }
 """
 
@@ -119,3 +130,39 @@ def test_special_operands_follow_ieee_division(device):
     assert ok.sum() == 64
     assert _same_bits(out[ok, 0], want[ok]).all()
     assert _same_bits(out[ok, 1], want[ok]).all()
+
+
+def _run_sqrt(x):
+    import ctypes
+    from pysweep_amd import hip
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = len(x)
+    mod = _module()
+    fn = mod.get_function("ps_sqrt_check")
+    dx, do = hip.DeviceArray(8 * n), hip.DeviceArray(16 * n)
+    dx.upload(x)
+    block = 256
+    grid = (n + block - 1) // block  # the kernel returns for i >= cnt
+    hip.launch(fn, (grid,), (block,), [ctypes.c_void_p(dx.ptr), ctypes.c_void_p(do.ptr), ctypes.c_int(n)])
+    hip.synchronize()
+    out = do.download(np.empty((n, 2)))
+    dx.close()
+    do.close()
+    mod.unload()
+    return out
+
+
+def test_square_root_matches_ieee(device):
+    rng = np.random.default_rng(11)
+    n = 1 << 20
+    x = np.abs(_random_normals(rng, n, -760, 1000))
+    x[: n // 4] = rng.uniform(1e-3, 20.0, n // 4)       # densities, sound speeds
+    x[n // 4: n // 4 + 16] = [0.0, -0.0, np.inf, -np.inf, np.nan, -1.0, -4.0, 1.0, 4.0, 2.0,
+                              0.5, 1e300, 1e-300, 2.0 ** -767, -(2.0 ** -767), 1.5]
+    out = _run_sqrt(x)
+    with np.errstate(invalid="ignore"):
+        want = np.sqrt(x)
+    covered = ~(np.isfinite(x) & (x > 0) & (x < 2.0 ** -767))
+    assert covered.sum() == n - 1  # only 1e-300 lies below the range
+    assert _same_bits(out[covered, 1], out[covered, 0]).all()
+    assert _same_bits(out[:, 0], want).all()  # and the GPU's sqrt is IEEE
