@@ -164,14 +164,20 @@ typedef struct ps_standard_plan {
  * core/functions.py:126-161). */
 int ps_standard_run(const ps_standard_plan *plan, int gts_begin, int gts_end);
 
-/* Per-kernel timing of a run: HIP events around every launch on the work
+/* Per-kernel timing of a run: HIP events around launches on the work
  * stream, resolved after the run (no counterpart in the reference, whose
  * only timing is wall clock around the whole solve, core/solver.py:13-68).
- * Index: PS_K_* below; ms are summed, launches counted (both accumulate). */
+ * Index: PS_K_* below; ms are summed over the timed launches, which are
+ * counted in `launches` (both accumulate).  `kinds` (a bit mask of
+ * 1 << PS_K_*, 0 = every kind) and `every` (time one launch in `every` of
+ * each timed kind, counted per run call; 0 or 1 = each) sample the
+ * launches, so that timing a long run adds few event records to it. */
 enum { PS_K_UP = 0, PS_K_YBRIDGE = 1, PS_K_XBRIDGE = 2, PS_K_OCTAHEDRON = 3, PS_K_DOWN = 4, PS_K_STANDARD = 5 };
 typedef struct ps_run_stats {
     double ms[6];
     long long launches[6];
+    int kinds;
+    int every;
 } ps_run_stats;
 int ps_swept_run_timed(const ps_swept_plan *plan, int G, int c_begin, int c_end, ps_run_stats *stats);
 int ps_standard_run_timed(const ps_standard_plan *plan, int gts_begin, int gts_end, ps_run_stats *stats);

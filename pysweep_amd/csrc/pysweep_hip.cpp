@@ -60,13 +60,14 @@ int launch(ps_function f, unsigned gx, unsigned threads, void **args)
     return 0;
 }
 
-// Optional per-kernel timing: an event pair around every launch, resolved
-// after the run (ps_run_stats in the header).
+// Optional per-kernel timing: an event pair around each sampled launch,
+// resolved after the run (ps_run_stats in the header).
 struct Timer {
     ps_run_stats *stats = nullptr;
-    std::vector<hipEvent_t> ev;   // 2 per launch
+    std::vector<hipEvent_t> ev;   // 2 per timed launch
     std::vector<int> kind;
     size_t used = 0;
+    long long seen[6] = {0, 0, 0, 0, 0, 0};  // launches of each kind in this run call
 };
 
 Timer g_timer;
@@ -76,12 +77,22 @@ int timer_begin(ps_run_stats *stats)
     g_timer.stats = stats;
     g_timer.used = 0;
     g_timer.kind.clear();
+    for (long long &n : g_timer.seen) n = 0;
     return 0;
+}
+
+// whether this launch of `kind` is one the caller asked to time
+bool timer_wants(int kind)
+{
+    const ps_run_stats *st = g_timer.stats;
+    if (!st) return false;
+    const long long n = g_timer.seen[kind]++;
+    if (st->kinds && !(st->kinds & (1 << kind))) return false;
+    return st->every <= 1 || n % st->every == 0;
 }
 
 int timer_mark(bool start, int kind)
 {
-    if (!g_timer.stats) return 0;
     if (start) {
         if (g_timer.used + 2 > g_timer.ev.size()) {
             for (int i = 0; i < 64; ++i) {
@@ -114,10 +125,11 @@ int timer_end()
 
 int launch_kind(ps_function f, unsigned gx, unsigned threads, void **args, int kind)
 {
-    int rc = timer_mark(true, kind);
+    const bool timed = timer_wants(kind);
+    int rc = timed ? timer_mark(true, kind) : 0;
     if (rc) return rc;
     if ((rc = launch(f, gx, threads, args))) return rc;
-    return timer_mark(false, kind);
+    return timed ? timer_mark(false, kind) : 0;
 }
 
 __global__ void ps_copy_kernel(double *__restrict__ dst, const double *__restrict__ src, size_t n)

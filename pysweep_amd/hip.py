@@ -62,8 +62,18 @@ class MultiPlan(ctypes.Structure):
 
 
 class RunStats(ctypes.Structure):
-    """Per-kernel launch time totals (index: KERNEL_KINDS)."""
-    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_longlong * 6)]
+    """Per-kernel launch time totals (index: KERNEL_KINDS) of the timed
+    launches: every launch by default, or one in ``every`` launches of the
+    kernels named in ``kinds``."""
+    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_longlong * 6), ("kinds", c_int),
+                ("every", c_int)]
+
+    @classmethod
+    def sampled(cls, kinds=(), every=1):
+        st = cls()
+        st.kinds = sum(1 << KERNEL_KINDS.index(k) for k in kinds)
+        st.every = every
+        return st
 
     def as_dict(self):
         return {k: {"ms": self.ms[i], "launches": self.launches[i],
