@@ -308,6 +308,22 @@ def short_first(running, now, capacity=1 << 20, max_wait=2.0):
     return out
 
 
+def oldest_first(running, capacity=1 << 20):
+    """Problems that take part in the next rounds: in arrival order while
+    their cells fit in ``capacity`` (about what fills the GPU), the oldest
+    always.  A large problem then runs nearly alone, as a first-come,
+    first-served server would run it, while small ones share its rounds only
+    as far as they fill the GPU; nothing starves, since the oldest problem
+    is always in."""
+    out, used = set(), 0
+    for t in sorted(running, key=lambda t: (t.arrival, t.admitted)):
+        if out and used + t.cells > capacity:
+            break
+        out.add(t)
+        used += t.cells
+    return out
+
+
 class Engine:
     """Continuous batching of swept solves on one GPU.
 
@@ -317,9 +333,11 @@ class Engine:
     latency, so it is not the default)."""
 
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
-                 poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True):
+                 poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
+                 capacity=1 << 20):
         hip.ensure_init(0)
         self.policy = policy
+        self.capacity = capacity  # cells per round for "oldest_first"
         self.solo = solo  # a problem running alone uses the plain kernels
         # host steps whose rounds may be queued on the GPU at once: admitting
         # a request waits for the queued rounds, so this bounds its delay
@@ -372,10 +390,12 @@ class Engine:
         next retirement; while requests that have not arrived yet are queued,
         as many as the measured round time fits before the next arrival (at
         least ``poll_rounds``), so they are admitted promptly; while some
-        running problems sit out, ``poll_rounds`` (the selection is revisited)."""
+        running problems sit out under "short_first", ``poll_rounds`` (its
+        selection changes with time; "oldest_first" changes only at
+        retirements and arrivals)."""
         r = min(t.c_end - t.c for t in active)
-        if len(active) < len(self.running):
-            r = min(r, self.poll_rounds)
+        if len(active) < len(self.running) and self.policy == "short_first":
+            r = min(r, self.poll_rounds)  # the selection changes with time
         elif self.queue:
             now = self.now()
             if any(t.arrival <= now for t in self.queue):  # waiting for memory: until a retirement
@@ -436,10 +456,16 @@ class Engine:
             if self.queue:
                 time.sleep(max(0.0, min(t.arrival for t in self.queue) - self.now()))
             return
-        if self.solo and len(self.running) == 1:
+        if self.policy == "short_first":
+            active = short_first(self.running, self.now())
+        elif self.policy == "oldest_first":
+            active = oldest_first(self.running, self.capacity)
+        else:
+            active = set(self.running)
+        if self.solo and len(active) == 1:
             # one problem: the plain kernels, as a one-at-a-time solve runs it
-            t = self.running[0]
-            R = self._rounds_ahead({t})
+            t = next(iter(active))
+            R = self._rounds_ahead(active)
             self._run_solo(t, R)
             self.tables[t.cls.key].dirty = True  # its round counter moved on the host only
             self.rounds += R
@@ -448,10 +474,6 @@ class Engine:
             if t.c >= t.c_end:
                 self._retire(t)
             return
-        if self.policy == "short_first":
-            active = short_first(self.running, self.now())
-        else:
-            active = set(self.running)
         R = self._rounds_ahead(active)
         for key, tab in self.tables.items():
             if tab.dirty or tab.needs(active):

@@ -177,3 +177,20 @@ def test_solo_path_equals_batched_rounds(device, method):
     b, _ = _run([p], solo=False)
     want = oracle_stages(step, p.ic, 2, 20)
     assert np.array_equal(a[0].result, want) and np.array_equal(b[0].result, want)
+
+
+@pytest.mark.parametrize("capacity", [1, 64 * 96 + 96 * 64, 1 << 19])
+def test_oldest_first_policy_matches_oracle(device, capacity):
+    """The "oldest_first" selection (rounds take problems in arrival order
+    up to a cell capacity; capacity 1 runs them one at a time on the solo
+    path, the middle one mixes solo and batched rounds) leaves every result
+    bitwise the oracle's."""
+    mix = _mix()
+    from pysweep_amd.batch import Engine
+    eng = Engine(policy="oldest_first", capacity=capacity)
+    tickets = [eng.submit(p, arrival=0.01 * i) for i, (p, _) in enumerate(mix)]
+    eng.run()
+    for t, (p, step) in zip(tickets, mix):
+        steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
+        want = oracle_stages(step, p.ic, p.its, p.its * steps)
+        assert np.array_equal(t.result, want), (p.plugin, p.ic.shape, capacity)

@@ -268,6 +268,30 @@ def test_prefetch_register_bound_covers_every_face_load():
         assert compiler._face_cells(b, k, s) == most, (b, k, s)
 
 
+def test_oldest_first_selection():
+    """The engine's "oldest_first" round selection: arrival order while the
+    cells fit the capacity, the oldest problem always in."""
+    from pysweep_amd import batch
+
+    def ticket(n, arrival, admitted=0.0):
+        return batch.Ticket(problem=None, cells=n * n, arrival=arrival, admitted=admitted)
+
+    big, s1, s2, s3 = ticket(2048, 0.0), ticket(256, 1.0), ticket(256, 2.0), ticket(512, 3.0)
+    cap = 1 << 19
+    # the oldest is in even though it alone exceeds the capacity; nothing else fits beside it
+    assert batch.oldest_first([s2, big, s1], cap) == {big}
+    # without it: s1, s2 (2 x 65536 cells) fit, s3 (262144) too, in arrival order
+    assert batch.oldest_first([s3, s2, s1], cap) == {s1, s2, s3}
+    assert batch.oldest_first([s3, s2, s1], 3 * 65536) == {s1, s2}
+    # selection stops at the first that does not fit (no skipping ahead of older problems)
+    huge = ticket(1024, 1.5)
+    assert batch.oldest_first([s3, huge, s2, s1], cap) == {s1}
+    # ties in arrival: admission order
+    a, b = ticket(128, 5.0, admitted=5.1), ticket(128, 5.0, admitted=5.0)
+    assert batch.oldest_first([a, b], 128 * 128) == {b}
+    assert batch.oldest_first([], cap) == set()
+
+
 def test_short_first_selection():
     """The engine's round selection (pysweep_amd.batch.short_first): the
     shortest remaining work first, packed up to the capacity in cells; a
