@@ -284,6 +284,60 @@ __device__ __forceinline__ int eu_div(int t, int w, float rw)
     return q;
 }
 
+/* exchange the two momentum components */
+__device__ __forceinline__ void eu_swap(eu_q &q)
+{
+    const double t = q.m;
+    q.m = q.n;
+    q.n = t;
+}
+
+/* One face of a level update between the cells at idx - stride and idx:
+ * flux in components 0-3 and dissipation in 4-7 of o (component stride
+ * PS_MAXFACES), P holding the cells' pressures.  A y face (stride 1) is
+ * evaluated as an x face with the momentum components exchanged in its
+ * states and exchanged back in its results; the only operations that then
+ * run in another order are commutative (m*n as n*m, m*m + n*n as
+ * n*n + m*m), so the values are bitwise those of eu_flux<false> and
+ * eu_dissipation<false>, and x and y faces share one instruction stream. */
+template <class R>
+__device__ __forceinline__ void eu_face(const R *cur, const double *P, double *o, int idx, int stride, bool ydir)
+{
+    constexpr int FS = PS_MAXFACES;
+    eu_q wq = eu_at(cur, idx - stride), c = eu_at(cur, idx);
+    const double pww = P[idx - 2 * stride], pw = P[idx - stride], pc = P[idx], pe = P[idx + stride];
+    if (ydir) {
+        eu_swap(wq);
+        eu_swap(c);
+    }
+    const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
+    const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
+    const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
+    eu_q f = eu_flux<true>(L, Rr, rl, rr), d = eu_dissipation<true>(L, Rr, rl, rr);
+    if (ydir) {
+        eu_swap(f);
+        eu_swap(d);
+    }
+    o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
+    o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
+}
+
+/* 0.5*((((0 + flux_a) + diss_a) - flux_b) - diss_b) per component, face a
+ * on the low side of the cell and face b on the high side: the per-cell
+ * step's order (eu_divergence) */
+__device__ __forceinline__ eu_q eu_face_diff(const double *fa, const double *fb)
+{
+    constexpr int FS = PS_MAXFACES;
+    eu_q a;
+    a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
+    a.r = a.r + fa[0]; a.m = a.m + fa[FS]; a.n = a.n + fa[2 * FS]; a.e = a.e + fa[3 * FS];
+    a.r = a.r + fa[4 * FS]; a.m = a.m + fa[5 * FS]; a.n = a.n + fa[6 * FS]; a.e = a.e + fa[7 * FS];
+    a.r = a.r - fb[0]; a.m = a.m - fb[FS]; a.n = a.n - fb[2 * FS]; a.e = a.e - fb[3 * FS];
+    a.r = a.r - fb[4 * FS]; a.m = a.m - fb[5 * FS]; a.n = a.n - fb[6 * FS]; a.e = a.e - fb[7 * FS];
+    a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
+    return a;
+}
+
 template <class R, int MAXC>
 __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
                                          int globalTimeStep, double *sc)
@@ -308,70 +362,48 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
     }
     __syncthreads();
 
-    /* 2. x faces: face i (0..h) lies west of cell x0+i */
+    /* 2. faces.  x face i (0..h) lies west of cell x0+i, at F[i*w + j]; y
+     * face j (0..w) lies south of cell y0+j, at Fy[i*(w+1) + j].  When both
+     * sets fit the buffer they are evaluated in one pass over one lane space
+     * (the small levels then fill whole wavefronts, and the level needs two
+     * barriers fewer); otherwise the x divergences are taken first and the
+     * y faces overwrite the x faces. */
+    const int nfx = (h + 1) * w, nfy = h * (w + 1), wf = w + 1;
+    const float rwf = 1.0f / (float)wf;
+    const bool merged = nfx + nfy <= FS;
+    double *Fy = merged ? F + nfx : F;
     {
-        const int n = (h + 1) * w;
+        const int n = merged ? nfx + nfy : nfx;
         for (int t = threadIdx.x; t < n; t += PS_NT) {
-            const int i = eu_div(t, w, rw), j = t - i * w;
-            const int x = x0 + i, y = y0 + j;
-            const int idx = x * SY + y;
-            const eu_q wq = eu_at(cur, idx - SY), c = eu_at(cur, idx);
-            const double pww = P[(x - 2) * SY + y], pw = P[(x - 1) * SY + y];
-            const double pc = P[x * SY + y], pe = P[(x + 1) * SY + y];
-            const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
-            const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
-            const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
-            const eu_q f = eu_flux<true>(L, Rr, rl, rr), d = eu_dissipation<true>(L, Rr, rl, rr);
-            double *o = F + t;
-            o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
-            o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
+            const bool xf = t < nfx;
+            const int u = xf ? t : t - nfx, wd = xf ? w : wf;
+            const int i = eu_div(u, wd, xf ? rw : rwf), j = u - i * wd;
+            eu_face(cur, P, F + t, (x0 + i) * SY + y0 + j, xf ? SY : 1, !xf);
         }
     }
     __syncthreads();
 
-    /* 3. x divergence of my cells (cell (i, j): west face i, east face i+1) */
     eu_q dfx[MAXC];
+    if (!merged) {
+        /* 3. x divergence of my cells (cell (i, j): west face i, east face i+1) */
 #pragma unroll
-    for (int m = 0; m < MAXC; ++m) {
-        const int t = threadIdx.x + m * PS_NT;
-        if (t < h * w) {
-            const int i = eu_div(t, w, rw), j = t - i * w;
-            const double *fw = F + (i * w + j), *fe = F + ((i + 1) * w + j);
-            eu_q a;
-            a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
-            a.r = a.r + fw[0]; a.m = a.m + fw[FS]; a.n = a.n + fw[2 * FS]; a.e = a.e + fw[3 * FS];
-            a.r = a.r + fw[4 * FS]; a.m = a.m + fw[5 * FS]; a.n = a.n + fw[6 * FS]; a.e = a.e + fw[7 * FS];
-            a.r = a.r - fe[0]; a.m = a.m - fe[FS]; a.n = a.n - fe[2 * FS]; a.e = a.e - fe[3 * FS];
-            a.r = a.r - fe[4 * FS]; a.m = a.m - fe[5 * FS]; a.n = a.n - fe[6 * FS]; a.e = a.e - fe[7 * FS];
-            a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
-            dfx[m] = a;
+        for (int m = 0; m < MAXC; ++m) {
+            const int t = threadIdx.x + m * PS_NT;
+            if (t < h * w) {
+                const int i = eu_div(t, w, rw), j = t - i * w;
+                dfx[m] = eu_face_diff(F + (i * w + j), F + ((i + 1) * w + j));
+            }
         }
-    }
-    __syncthreads();
-
-    /* 4. y faces: face j (0..w) lies south of cell y0+j */
-    {
-        const int wf = w + 1, n = h * wf;
-        const float rwf = 1.0f / (float)wf;
-        for (int t = threadIdx.x; t < n; t += PS_NT) {
+        __syncthreads();
+        /* 4. y faces over the x faces */
+        for (int t = threadIdx.x; t < nfy; t += PS_NT) {
             const int i = eu_div(t, wf, rwf), j = t - i * wf;
-            const int x = x0 + i, y = y0 + j;
-            const int idx = x * SY + y;
-            const eu_q wq = eu_at(cur, idx - 1), c = eu_at(cur, idx);
-            const double pww = P[x * SY + y - 2], pw = P[x * SY + y - 1];
-            const double pc = P[x * SY + y], pe = P[x * SY + y + 1];
-            const eu_q L = eu_limit(wq, c, pc - pw, pw - pww);
-            const eu_q Rr = eu_limit(c, wq, pc - pw, pe - pc);
-            const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
-            const eu_q f = eu_flux<false>(L, Rr, rl, rr), d = eu_dissipation<false>(L, Rr, rl, rr);
-            double *o = F + t;
-            o[0] = f.r; o[FS] = f.m; o[2 * FS] = f.n; o[3 * FS] = f.e;
-            o[4 * FS] = d.r; o[5 * FS] = d.m; o[6 * FS] = d.n; o[7 * FS] = d.e;
+            eu_face(cur, P, Fy + t, (x0 + i) * SY + y0 + j, 1, true);
         }
+        __syncthreads();
     }
-    __syncthreads();
 
-    /* 5. y divergence and the RK2 update of my cells */
+    /* 5. (x and) y divergence and the RK2 update of my cells */
     const bool fin = globalTimeStep % 2 == 0;
     const double hh = fin ? 1.0 : 0.5;
 #pragma unroll
@@ -379,15 +411,9 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
         const int t = threadIdx.x + m * PS_NT;
         if (t < h * w) {
             const int i = eu_div(t, w, rw), j = t - i * w;
-            const double *fs = F + (i * (w + 1) + j), *fn = fs + 1;
-            eu_q a;
-            a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
-            a.r = a.r + fs[0]; a.m = a.m + fs[FS]; a.n = a.n + fs[2 * FS]; a.e = a.e + fs[3 * FS];
-            a.r = a.r + fs[4 * FS]; a.m = a.m + fs[5 * FS]; a.n = a.n + fs[6 * FS]; a.e = a.e + fs[7 * FS];
-            a.r = a.r - fn[0]; a.m = a.m - fn[FS]; a.n = a.n - fn[2 * FS]; a.e = a.e - fn[3 * FS];
-            a.r = a.r - fn[4 * FS]; a.m = a.m - fn[5 * FS]; a.n = a.n - fn[6 * FS]; a.e = a.e - fn[7 * FS];
-            a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
-            const eu_q fx = dfx[m];
+            const eu_q fx = merged ? eu_face_diff(F + (i * w + j), F + ((i + 1) * w + j)) : dfx[m];
+            const double *fs = Fy + (i * wf + j);
+            const eu_q a = eu_face_diff(fs, fs + 1);
             const int idx = (x0 + i) * SY + (y0 + j);
             const eu_q b = eu_at(fin ? prev : cur, idx);
             R *o = out + idx;
