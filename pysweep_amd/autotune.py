@@ -15,9 +15,11 @@ from pysweep_amd import compiler
 
 DEFAULT_PATH = os.path.join(compiler.CACHE, "tuning.json")
 
-#: defaults from MI355X measurements of the Euler vortex (1024^2, 500 steps):
-#: b=16 with a 5-level window ran 3.7 GCUPS, b=32 with 4 levels and 512
-#: threads 4.1 GCUPS (single-problem launches).
+#: defaults from MI355X measurements of single-problem swept launches: the
+#: Euler vortex at 2048^2 ran 7.6 GCUPS with b=16 against 6.6 with b=32 (the
+#: b=32 tile's 126 KB of LDS leaves one workgroup per CU); heat 8192^2 ran
+#: 178 GCUPS with b=32 and 271 with b=64, which only grids that are
+#: multiples of 64 can use.
 DEFAULTS = {
     ("euler", "fp64"): {"b": 16, "grid_per_cu": 0},
     ("euler", "fp32"): {"b": 16, "grid_per_cu": 0},
