@@ -205,7 +205,7 @@ HEAT_FLOPS_PER_CELL_STAGE = 12      # heat.cu:11-24 as written
 
 def large_instances():
     """Large single instances, 1000 steps, swept vs standard, FP64 and FP32
-    state: heat 8192^2 (b=32, and b=64) and Euler 2048^2 (b=32).  FP32
+    state: heat 8192^2 (b=32, and b=64) and Euler 2048^2 (b=32, and b=16).  FP32
     errors are stated against the FP64 result (itself bitwise equal to the
     CPU oracle, which the parity tests check at smaller sizes)."""
     import numpy as np
@@ -222,8 +222,8 @@ def large_instances():
     n = 2048
     dx = 10.0 / n
     dt = 0.1 * dx
-    cases.append(("euler", n, (32,), 4, 2, 2, [0.0, 1000 * dt, dt, dx, dx, 1.4], ("euler.cu", "euler_fp32.cu",
-                                                                                 "euler.py")))
+    cases.append(("euler", n, (32, 16), 4, 2, 2, [0.0, 1000 * dt, dt, dx, dx, 1.4],
+                  ("euler.cu", "euler_fp32.cu", "euler.py")))
     for name, n, bs, nv, k, s, g, (c64, c32, py) in cases:
         ic = workloads.heat_mode(n, n) if name == "heat" else workloads.vortex_state(n, n)
         cpu = plugins.load_cpu_module(eq_path(py))
