@@ -271,8 +271,13 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  * out may be prev's own slot (a swept window of ITS levels suffices) */
 #define PS_LEVEL_INPLACE 1
 
-/* t / w for 0 <= t < 2^22, w >= 1, rw = 1.0f / w: the float quotient is
- * within one of the true one; one correction makes it exact */
+/* 1/w for w >= 1 within one ulp: the hardware reciprocal (one instruction
+ * where a correctly rounded float division takes ten) */
+__device__ __forceinline__ float eu_rcpf(int w) { return __builtin_amdgcn_rcpf((float)w); }
+
+/* t / w for 0 <= t < 2^22, w >= 1, rw within one ulp of 1/w (the hardware
+ * reciprocal, eu_rcpf): the float quotient is within one of the true one;
+ * one correction makes it exact */
 __device__ __forceinline__ int eu_div(int t, int w, float rw)
 {
     int q = (int)((float)t * rw);
@@ -343,7 +348,7 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
                                          int globalTimeStep, double *sc)
 {
     const int h = x1 - x0, w = y1 - y0;
-    const float rw = 1.0f / (float)w;
+    const float rw = eu_rcpf(w);
     constexpr int FS = PS_MAXFACES;  /* component stride of the face buffer */
     double *P = sc;                  /* pressures, tile-indexed x*SY + y */
     double *F = sc + SX * SY;        /* face f, component c at F[c*FS + f]: flux 0-3, dissipation 4-7 */
@@ -351,7 +356,7 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
     /* 1. pressures on the cross-shaped footprint */
     {
         const int bh = h + 4, bw = w + 4, n = bh * bw;
-        const float rbw = 1.0f / (float)bw;
+        const float rbw = eu_rcpf(bw);
         for (int t = threadIdx.x; t < n; t += PS_NT) {
             const int i = eu_div(t, bw, rbw), j = t - i * bw;
             const bool inx = i >= 2 && i < h + 2, iny = j >= 2 && j < w + 2;
@@ -369,7 +374,7 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
      * barriers fewer); otherwise the x divergences are taken first and the
      * y faces overwrite the x faces. */
     const int nfx = (h + 1) * w, nfy = h * (w + 1), wf = w + 1;
-    const float rwf = 1.0f / (float)wf;
+    const float rwf = eu_rcpf(wf);
     const bool merged = nfx + nfy <= FS;
     double *Fy = merged ? F + nfx : F;
     {

@@ -63,7 +63,8 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  * measured faster there) */
 #define PS_LEVEL_SWEPT_ONLY 1
 
-/* t / w for 0 <= t < 2^22, w >= 1, rw = 1.0f / w (one correction) */
+/* t / w for 0 <= t < 2^22, w >= 1, rw within one ulp of 1/w (the hardware
+ * reciprocal): the float quotient is within one of t / w; one correction */
 __device__ __forceinline__ int heat_div(int t, int w, float rw)
 {
     int q = (int)((float)t * rw);
@@ -80,7 +81,7 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
                                          int globalTimeStep, double *)
 {
     const int w = y1 - y0, n = (x1 - x0) * w;
-    const float rw = 1.0f / (float)w;
+    const float rw = __builtin_amdgcn_rcpf((float)w);
     const bool fin = globalTimeStep % 2 == 0;
     for (int t = threadIdx.x; t < n; t += PS_NT) {
         const int i = heat_div(t, w, rw);

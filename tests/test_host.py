@@ -231,16 +231,21 @@ def test_cpu_plugin_halves_follow_oracle(name):
 
 
 def test_float_reciprocal_division_is_exact():
-    """The kernels' ps_div: q = int(float(t) * (1.0f / w)) corrected once is
-    t // w for 0 <= t < 2^22 (restated in float32 arithmetic)."""
+    """The kernels' ps_div / eu_div / heat_div: q = int(float(t) * rw)
+    corrected once is t // w for 0 <= t < 2^22, for any rw within one ulp of
+    1/w (the hardware reciprocal the kernels use is accurate to one ulp);
+    restated in float32 arithmetic with the correctly rounded reciprocal and
+    its two float neighbours."""
     rng = np.random.default_rng(3)
     for w in list(range(1, 300)) + [1023, 1024, 1025, 4096, 65535]:
-        t = np.concatenate([np.arange(0, 4 * w + 3), rng.integers(0, 1 << 22, 2000)]).astype(np.int64)
-        rw = np.float32(1.0) / np.float32(w)
-        q = (t.astype(np.float32) * rw).astype(np.int64)
-        r = t - q * w
-        q = np.where(r < 0, q - 1, np.where(r >= w, q + 1, q))
-        assert np.array_equal(q, t // w), w
+        t = np.concatenate([np.arange(0, 4 * w + 3), (1 << 22) - 1 - np.arange(4 * w),
+                            rng.integers(0, 1 << 22, 2000)]).astype(np.int64)
+        r0 = np.float32(1.0) / np.float32(w)
+        for rw in (r0, np.nextafter(r0, np.float32(0)), np.nextafter(r0, np.float32(1))):
+            q = (t.astype(np.float32) * rw).astype(np.int64)
+            r = t - q * w
+            q = np.where(r < 0, q - 1, np.where(r >= w, q + 1, q))
+            assert np.array_equal(q, t // w), (w, rw)
 
 
 def test_prefetch_register_bound_covers_every_face_load():
