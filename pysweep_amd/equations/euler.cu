@@ -263,7 +263,11 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  *          rectangle the harness passes ((x1-x0+1)*(y1-y0) or
  *          (x1-x0)*(y1-y0+1)); the face buffer is component-major so that
  *          consecutive lanes touch consecutive LDS words;
- *   MAXC   upper bound of cells per lane, ceil(|S| / PS_NT).
+ *   MAXC   upper bound of cells per lane, ceil(|S| / PS_NT);
+ *   emit   called as emit(x, y, v) with the tile coordinates and the NV new
+ *          values of every cell of S, by the lane that computed it (the
+ *          kernels write the cell to HBM from there, without re-reading
+ *          LDS).
  */
 #define PS_HAS_LEVEL 1
 #define PS_LEVEL_SCRATCH(CELLS, FACES) ((CELLS) + 8 * (FACES))
@@ -343,9 +347,9 @@ __device__ __forceinline__ eu_q eu_face_diff(const double *fa, const double *fb)
     return a;
 }
 
-template <class R, int MAXC>
+template <class R, int MAXC, class Emit>
 __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
-                                         int globalTimeStep, double *sc)
+                                         int globalTimeStep, double *sc, const Emit &emit)
 {
     const int h = x1 - x0, w = y1 - y0;
     const float rw = eu_rcpf(w);
@@ -421,18 +425,24 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
             const eu_q a = eu_face_diff(fs, fs + 1);
             const int idx = (x0 + i) * SY + (y0 + j);
             const eu_q b = eu_at(fin ? prev : cur, idx);
-            R *o = out + idx;
+            R v[4];
             if (fin) {
-                o[0] = (R)(b.r + (fx.r * DTDX + a.r * DTDY));
-                o[VARS] = (R)(b.m + (fx.m * DTDX + a.m * DTDY));
-                o[2 * VARS] = (R)(b.n + (fx.n * DTDX + a.n * DTDY));
-                o[3 * VARS] = (R)(b.e + (fx.e * DTDX + a.e * DTDY));
+                v[0] = (R)(b.r + (fx.r * DTDX + a.r * DTDY));
+                v[1] = (R)(b.m + (fx.m * DTDX + a.m * DTDY));
+                v[2] = (R)(b.n + (fx.n * DTDX + a.n * DTDY));
+                v[3] = (R)(b.e + (fx.e * DTDX + a.e * DTDY));
             } else {
-                o[0] = (R)(b.r + hh * (fx.r * DTDX + a.r * DTDY));
-                o[VARS] = (R)(b.m + hh * (fx.m * DTDX + a.m * DTDY));
-                o[2 * VARS] = (R)(b.n + hh * (fx.n * DTDX + a.n * DTDY));
-                o[3 * VARS] = (R)(b.e + hh * (fx.e * DTDX + a.e * DTDY));
+                v[0] = (R)(b.r + hh * (fx.r * DTDX + a.r * DTDY));
+                v[1] = (R)(b.m + hh * (fx.m * DTDX + a.m * DTDY));
+                v[2] = (R)(b.n + hh * (fx.n * DTDX + a.n * DTDY));
+                v[3] = (R)(b.e + hh * (fx.e * DTDX + a.e * DTDY));
             }
+            R *o = out + idx;
+            o[0] = v[0];
+            o[VARS] = v[1];
+            o[2 * VARS] = v[2];
+            o[3 * VARS] = v[3];
+            emit(x0 + i, y0 + j, v);
         }
     }
 }

@@ -53,7 +53,8 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  * (row-major, y fastest).  cur is level r-1, prev level r-2 (the base of a
  * final RK2 stage), out level r, all with the tile strides; the arithmetic is
  * heat_step's, so results are bitwise the same.  Separate level pointers let
- * the swept kernels keep their LDS window circular (no copies).
+ * the swept kernels keep their LDS window circular (no copies).  Each new
+ * value is also handed to emit(x, y, values), which writes it to HBM.
  */
 #define PS_HAS_LEVEL 1
 #define PS_LEVEL_SCRATCH(CELLS, FACES) 1
@@ -76,9 +77,9 @@ __device__ __forceinline__ int heat_div(int t, int w, float rw)
     return q;
 }
 
-template <class R, int MAXC>
+template <class R, int MAXC, class Emit>
 __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
-                                         int globalTimeStep, double *)
+                                         int globalTimeStep, double *, const Emit &emit)
 {
     const int w = y1 - y0, n = (x1 - x0) * w;
     const float rw = __builtin_amdgcn_rcpf((float)w);
@@ -87,11 +88,14 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
         const int i = heat_div(t, w, rw);
         const int idx = (x0 + i) * SY + y0 + (t - i * w);
         const double d = heat_laplace(cur, idx);
+        R v[1];
         if (SCHEME)
-            out[idx] = (R)(d + (double)cur[idx]);
+            v[0] = (R)(d + (double)cur[idx]);
         else if (fin)
-            out[idx] = (R)(1.0 * d + (double)prev[idx]);
+            v[0] = (R)(1.0 * d + (double)prev[idx]);
         else
-            out[idx] = (R)(0.5 * d + (double)cur[idx]);
+            v[0] = (R)(0.5 * d + (double)cur[idx]);
+        out[idx] = v[0];
+        emit(x0 + i, y0 + (t - i * w), v);
     }
 }
