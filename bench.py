@@ -203,11 +203,12 @@ def batched_workload(n, seed=0):
 HEAT_FLOPS_PER_CELL_STAGE = 12      # heat.cu:11-24 as written
 
 
-def large_instances():
+def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float")):
     """Large single instances, 1000 steps, swept vs standard, FP64 and FP32
     state: heat 8192^2 (b=32, and b=64) and Euler 2048^2 (b=32, and b=16).  FP32
     errors are stated against the FP64 result (itself bitwise equal to the
-    CPU oracle, which the parity tests check at smaller sizes)."""
+    CPU oracle, which the parity tests check at smaller sizes).  ``names``,
+    ``steps`` and ``reals`` narrow the study (the profiling runs use them)."""
     import numpy as np
 
     from pysweep_amd import device, hip, plugins, schedule, workloads
@@ -217,30 +218,34 @@ def large_instances():
     n = 8192
     dx = 1.0 / n
     dt = 0.1 * dx * dx
-    heat_g = [0.0, 1000 * dt, dt, dx, dx, 1.0, True]
+    heat_g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
     cases.append(("heat", n, (32, 64), 1, 1, 1, heat_g, ("heat.cu", "heat_fp32.cu", "heat.py")))
     n = 2048
     dx = 10.0 / n
     dt = 0.1 * dx
-    cases.append(("euler", n, (32, 16), 4, 2, 2, [0.0, 1000 * dt, dt, dx, dx, 1.4],
+    cases.append(("euler", n, (32, 16), 4, 2, 2, [0.0, steps * dt, dt, dx, dx, 1.4],
                   ("euler.cu", "euler_fp32.cu", "euler.py")))
     for name, n, bs, nv, k, s, g, (c64, c32, py) in cases:
+        if name not in names:
+            continue
         ic = workloads.heat_mode(n, n) if name == "heat" else workloads.vortex_state(n, n)
         cpu = plugins.load_cpu_module(eq_path(py))
         cpu.set_globals(*g)
         finals = {}
         for real, cu in (("double", c64), ("float", c32)):
+            if real not in reals:
+                continue
             runs = [("standard", None)] + [("swept", b) for b in bs]
             for method, b in runs:
                 if method == "swept":
                     r = device.SweptRun(eq_path(cu), nv, k, s, n, n, b, real=real)
-                    times = schedule.swept_times(1000, b, k, s, exact=True)
+                    times = schedule.swept_times(steps, b, k, s, exact=True)
                     go = (lambda st, r=r, times=times: r.run(times, stats=st))
                     last = times.stages + times.tail + s - 1
                 else:
                     r = device.StandardRun(eq_path(cu), nv, k, s, n, n, real=real)
-                    go = (lambda st, r=r: r.run(1, s * 1000 + 1, stats=st))
-                    last = s * 1000 + s - 1
+                    go = (lambda st, r=r: r.run(1, s * steps + 1, stats=st))
+                    last = s * steps + s - 1
                 r.set_globals(cpu, g)
                 r.keep_ic_on_device(ic)
                 r.reset()
@@ -265,18 +270,18 @@ def large_instances():
                 flops = FLOPS_PER_CELL_STAGE if name == "euler" else HEAT_FLOPS_PER_CELL_STAGE
                 avg = d[kern]["avg_ms"] / 1e3
                 rec = {"case": f"{name}-{n}x{n}", "method": method, "b": b, "state": "fp64" if rb == 8 else "fp32",
-                       "gcups": n * n * 1000 / sec / 1e9, "seconds": sec, "dominant_kernel": kern,
+                       "gcups": n * n * steps / sec / 1e9, "seconds": sec, "dominant_kernel": kern,
                        "dominant_avg_launch_ms": d[kern]["avg_ms"],
                        "dominant_fp64_tflops": cs * flops / avg / 1e12,
                        "dominant_fp64_frac": cs * flops / avg / 1e12 / FP64_PEAK_TFLOPS,
                        # compulsory HBM bytes per step: heat 16 (read + write), Euler 160
                        # (stage 1: 32 R + 32 W; stage 2: 64 R + 32 W), halved for FP32 state
-                       "algorithmic_gbs": n * n * 1000 * (16 if name == "heat" else 160) * rb / 8 / sec / 1e9}
+                       "algorithmic_gbs": n * n * steps * (16 if name == "heat" else 160) * rb / 8 / sec / 1e9}
                 rec["algorithmic_hbm_frac"] = rec["algorithmic_gbs"] / HBM_PEAK_GBS
                 out.append(rec)
                 r.close()
-        ref = finals[("double", "standard", None)]
-        for key, f in finals.items():
+        ref = finals.get(("double", "standard", None))
+        for key, f in finals.items() if ref is not None else ():
             if key[0] == "double":
                 if not np.array_equal(f, ref):
                     out.append({"case": f"{name}-{n}x{n}", "mismatch": list(map(str, key))})
