@@ -327,14 +327,22 @@ def oldest_first(running, capacity=1 << 20):
 class Engine:
     """Continuous batching of swept solves on one GPU.
 
-    ``policy``: "all" (default: every running problem in every round) or
-    "short_first" (see ``short_first``; measured on MI355X it lowered the
-    served stream's throughput by a fifth without improving its median
-    latency, so it is not the default)."""
+    ``policy``: "all" (default: every running problem in every round),
+    "oldest_first" (arrival order up to ``capacity`` cells; see
+    ``oldest_first``) or "short_first" (see ``short_first``; measured on
+    MI355X it lowered the served stream's throughput by a fifth without
+    improving its median latency, so it is not the default).
 
-    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
+    ``use_graphs``: replay each table's round from a captured hipGraph (for
+    runs of at least ``graph_min_rounds`` rounds) instead of launching the
+    rounds from the library's C loop.  Off by default: measured on MI355X
+    the C loop already amortises launch cost and the replay was 1-4 %
+    slower (saturated served stream 16.8-17.0 against 17.0-17.2 solves/s;
+    512-problem mix 188-193 against 192-195)."""
+
+    def __init__(self, hbm_budget=None, max_active=None, use_graphs=False, cus=256, tuner=None, clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
-                 capacity=1 << 20):
+                 capacity=1 << 20, graph_min_rounds=8):
         hip.ensure_init(0)
         self.policy = policy
         self.capacity = capacity  # cells per round for "oldest_first"
@@ -348,6 +356,7 @@ class Engine:
         self.hbm_budget = hbm_budget if hbm_budget is not None else 64 << 30
         self.max_active = max_active
         self.use_graphs = use_graphs
+        self.graph_min_rounds = graph_min_rounds
         self.poll_rounds = poll_rounds
         self.max_rounds_ahead = max_rounds_ahead
         self.cus = cus
@@ -480,7 +489,10 @@ class Engine:
                 tab.rebuild(self.cus, active)
             if tab.plan is None or not any(tab.flags):
                 continue
-            if self.use_graphs:
+            if self.use_graphs and (tab.graph is not None or R >= self.graph_min_rounds):
+                # a graph is captured once per table layout and replayed; a
+                # layout about to change again within a few rounds runs its
+                # rounds eagerly instead (capturing costs more than it saves)
                 if tab.graph is None:
                     g = hip.Graph()
                     with g:
