@@ -333,16 +333,16 @@ class Engine:
     MI355X it lowered the served stream's throughput by a fifth without
     improving its median latency, so it is not the default).
 
-    ``use_graphs``: replay each table's round from a captured hipGraph (for
-    runs of at least ``graph_min_rounds`` rounds) instead of launching the
-    rounds from the library's C loop.  Off by default: measured on MI355X
+    ``use_graphs``: replay each table's rounds from a captured hipGraph of
+    ``graph_rounds`` rounds instead of launching them from the library's C
+    loop.  Off by default: measured on MI355X
     the C loop already amortises launch cost and the replay was 1-4 %
     slower (saturated served stream 16.8-17.0 against 17.0-17.2 solves/s;
     512-problem mix 188-193 against 192-195)."""
 
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=False, cus=256, tuner=None, clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
-                 capacity=1 << 20, graph_min_rounds=8):
+                 capacity=1 << 20, graph_rounds=8):
         hip.ensure_init(0)
         self.policy = policy
         self.capacity = capacity  # cells per round for "oldest_first"
@@ -356,7 +356,7 @@ class Engine:
         self.hbm_budget = hbm_budget if hbm_budget is not None else 64 << 30
         self.max_active = max_active
         self.use_graphs = use_graphs
-        self.graph_min_rounds = graph_min_rounds
+        self.graph_rounds = graph_rounds
         self.poll_rounds = poll_rounds
         self.max_rounds_ahead = max_rounds_ahead
         self.cus = cus
@@ -489,17 +489,20 @@ class Engine:
                 tab.rebuild(self.cus, active)
             if tab.plan is None or not any(tab.flags):
                 continue
-            if self.use_graphs and (tab.graph is not None or R >= self.graph_min_rounds):
-                # a graph is captured once per table layout and replayed; a
-                # layout about to change again within a few rounds runs its
-                # rounds eagerly instead (capturing costs more than it saves)
+            K = self.graph_rounds
+            if self.use_graphs and (tab.graph is not None or R >= K):
+                # a graph of K rounds is captured once per table layout and
+                # replayed; the remainder (and a layout about to change
+                # again within K rounds) runs eagerly
                 if tab.graph is None:
                     g = hip.Graph()
                     with g:
-                        hip.multi_run(tab.plan, 1)
+                        hip.multi_run(tab.plan, K)
                     tab.graph = g
-                for _ in range(R):
+                for _ in range(R // K):
                     tab.graph.launch()
+                if R % K:
+                    hip.multi_run(tab.plan, R % K)
             else:
                 hip.multi_run(tab.plan, R)
         self.rounds += R

@@ -64,9 +64,12 @@ def test_alone_equals_in_mix(device):
         assert np.array_equal(alone[0].result, tickets[i].result)
 
 
-def test_graph_replay_equals_eager(device):
+@pytest.mark.parametrize("rounds", [1, 3])
+def test_graph_replay_equals_eager(device, rounds):
+    """Rounds replayed from captured hipGraphs of 1 or 3 rounds (remainders
+    eager) give the same bits as eager rounds."""
     mix = _mix()
-    a, _ = _run([p for p, _ in mix], use_graphs=True)
+    a, _ = _run([p for p, _ in mix], use_graphs=True, graph_rounds=rounds)
     b, _ = _run([p for p, _ in mix], use_graphs=False)
     for x, y in zip(a, b):
         assert np.array_equal(x.result, y.result)
