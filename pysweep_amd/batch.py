@@ -334,13 +334,14 @@ class Engine:
     improving its median latency, so it is not the default).
 
     ``use_graphs``: replay each table's rounds from a captured hipGraph of
-    ``graph_rounds`` rounds instead of launching them from the library's C
-    loop.  Off by default: measured on MI355X
-    the C loop already amortises launch cost and the replay was 1-4 %
-    slower (saturated served stream 16.8-17.0 against 17.0-17.2 solves/s;
-    512-problem mix 188-193 against 192-195)."""
+    ``graph_rounds`` rounds (the remainder, and runs shorter than that,
+    launched from the library's C loop).  Measured on MI355X: one-round
+    graphs replayed from Python were 1-2 % slower than the C loop on the
+    saturated served stream; graphs of 8 or 32 rounds run level with it
+    (17.05-17.12 solves/s against 16.97-17.10; 512-problem mix 193-195
+    against 191-195)."""
 
-    def __init__(self, hbm_budget=None, max_active=None, use_graphs=False, cus=256, tuner=None, clock=None,
+    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
                  capacity=1 << 20, graph_rounds=8):
         hip.ensure_init(0)
