@@ -296,6 +296,45 @@ def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float
     return out
 
 
+def precision_sensitivity(n=1024, b=16, steps=(10, 100, 1000)):
+    """Euler vortex n^2 (swept b): the FP32-state solve's error against the
+    FP64 solve, beside the FP64 solve's own change when every value of the
+    initial condition moves by one ulp.  The two grow alike: the limiter
+    amplifies any perturbation, however small, to the same level."""
+    import numpy as np
+
+    from pysweep_amd import device, hip, plugins, schedule, workloads
+    from pysweep_amd.equations import path as eq_path
+    ic = workloads.vortex_state(n, n)
+    ulp = np.nextafter(ic, np.where(ic >= 0, np.inf, -np.inf))
+    out = []
+    for st in steps:
+        dx = 10.0 / n
+        dt = 0.1 * dx
+        g = [0.0, st * dt, dt, dx, dx, 1.4]
+        cpu = plugins.load_cpu_module(eq_path("euler.py"))
+        cpu.set_globals(*g)
+        times = schedule.swept_times(st, b, 2, 2, exact=True)
+        res = {}
+        for name, cu, real, init in (("fp64", "euler.cu", "double", ic), ("fp32", "euler_fp32.cu", "float", ic),
+                                     ("fp64_ulp", "euler.cu", "double", ulp)):
+            r = device.SweptRun(eq_path(cu), 4, 2, 2, n, n, b, real=real)
+            r.set_globals(cpu, g)
+            r.upload_ic(init)
+            last = r.run(times)
+            hip.synchronize()
+            res[name] = r.level(last)
+            r.close()
+        ref = res["fp64"]
+        rec = {"case": f"euler-{n}x{n}-b{b}", "steps": st}
+        for k in ("fp32", "fp64_ulp"):
+            e = res[k] - ref
+            rec[k + "_rel_L2"] = float(np.sqrt((e * e).sum() / (ref * ref).sum()))
+            rec[k + "_rel_Linf"] = float(np.abs(e).max() / np.abs(ref).max())
+        out.append(rec)
+    return out
+
+
 def heat_cpu_case():
     """The reference's CPU-runnable case: heat 384^2, b=8, forward Euler,
     100 steps (99 stages with the reference's quantization), swept, from
@@ -473,6 +512,7 @@ def main():
     st.close()
     if a.large:
         out["large_instances"] = large_instances()
+        out["precision_sensitivity"] = precision_sensitivity()
     if not a.no_cpu:
         out["heat_cpu_case"] = heat_cpu_case()
     if a.batch:

@@ -366,3 +366,40 @@ def test_in_place_window(device, name, b, slots):
     hip.synchronize()
     assert np.array_equal(run.level(last), oracle_stages(step, ic, 2, 2 * steps))
     run.close()
+
+
+def test_fp32_state_error_is_the_schemes_one_ulp_sensitivity(device):
+    """The FP32-state solve's error against FP64 is of the size the FP64
+    solve itself moves when every initial value moves by one ulp: the
+    limiter amplifies any perturbation (Euler vortex 128^2, 60 steps, swept
+    b=16).  So no storage precision below FP64 can meet a 1e-5 bound here."""
+    from pysweep_amd import device as dev, hip, plugins, schedule, workloads
+    from pysweep_amd.equations import path as eq_path
+    n, steps, b = 128, 60, 16
+    ic = workloads.vortex_state(n, n)
+    ulp = np.nextafter(ic, np.where(ic >= 0, np.inf, -np.inf))
+    dx = 10.0 / n
+    dt = 0.1 * dx
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    cpu = plugins.load_cpu_module(eq_path("euler.py"))
+    cpu.set_globals(*g)
+    times = schedule.swept_times(steps, b, 2, 2, exact=True)
+    res = {}
+    for name, cu, real, init in (("fp64", "euler.cu", "double", ic), ("fp32", "euler_fp32.cu", "float", ic),
+                                 ("ulp", "euler.cu", "double", ulp)):
+        r = dev.SweptRun(eq_path(cu), 4, 2, 2, n, n, b, real=real)
+        r.set_globals(cpu, g)
+        r.upload_ic(init)
+        last = r.run(times)
+        hip.synchronize()
+        res[name] = r.level(last)
+        r.close()
+    ref = res["fp64"]
+
+    def rel(x):
+        e = x - ref
+        return float(np.sqrt((e * e).sum() / (ref * ref).sum()))
+
+    e32, eulp = rel(res["fp32"]), rel(res["ulp"])
+    assert eulp > 1e-4          # one ulp of the input grows past 1e-4 in 60 steps
+    assert 0.2 < e32 / eulp < 10.0
