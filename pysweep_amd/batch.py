@@ -329,9 +329,12 @@ class Engine:
 
     ``policy``: "all" (default: every running problem in every round),
     "oldest_first" (arrival order up to ``capacity`` cells; see
-    ``oldest_first``) or "short_first" (see ``short_first``; measured on
-    MI355X it lowered the served stream's throughput by a fifth without
-    improving its median latency, so it is not the default).
+    ``oldest_first``) or "short_first" (least remaining work first up to
+    ``capacity`` cells, with aging; see ``short_first``).  Measured on
+    MI355X on a Poisson stream of Euler solves: "oldest_first" matched the
+    one-at-a-time server's p50 and p99; "short_first" lowered the mean
+    latency below it but raised the p99; "all" had the best saturated
+    throughput and the worst p99.
 
     ``use_graphs``: replay each table's rounds from a captured hipGraph of
     ``graph_rounds`` rounds (the remainder, and runs shorter than that,
@@ -346,7 +349,7 @@ class Engine:
                  capacity=1 << 20, graph_rounds=8):
         hip.ensure_init(0)
         self.policy = policy
-        self.capacity = capacity  # cells per round for "oldest_first"
+        self.capacity = capacity  # cells per round for "oldest_first" and "short_first"
         self.solo = solo  # a problem running alone uses the plain kernels
         # host steps whose rounds may be queued on the GPU at once: admitting
         # a request waits for the queued rounds, so this bounds its delay
@@ -467,7 +470,7 @@ class Engine:
                 time.sleep(max(0.0, min(t.arrival for t in self.queue) - self.now()))
             return
         if self.policy == "short_first":
-            active = short_first(self.running, self.now())
+            active = short_first(self.running, self.now(), capacity=self.capacity)
         elif self.policy == "oldest_first":
             active = oldest_first(self.running, self.capacity)
         else:
