@@ -112,6 +112,22 @@ def test_non_square_grid(device):
     assert np.array_equal(std.final_state, want)
 
 
+@pytest.mark.parametrize("nx,ny", [(48, 80), (80, 48), (16, 16)])
+def test_tile_counts_not_multiple_of_xcds(device, nx, ny):
+    """Tile counts that are not multiples of the 8 XCDs (15 and 1 swept b=16
+    tiles, 30 and 2 standard 8x16 tiles): the XCD-aware tile order still
+    visits every tile once."""
+    steps = 5
+    dx = 10.0 / nx
+    dt = 0.1 * dx
+    X, Y = np.meshgrid(o_euler.grid(nx), np.linspace(-5, 5, ny, endpoint=False), indexing="ij")
+    ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=3.0)
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
+    assert np.array_equal(gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True).final_state, want)
+    assert np.array_equal(gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True).final_state, want)
+
+
 # ---------------------------------------------------------------- known answers
 
 def _kat_globals(steps, scheme=None):
