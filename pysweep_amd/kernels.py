@@ -168,7 +168,8 @@ def default_slots(nv, its, b, P, real_bytes=8, scratch=0, least=None):
     windows slide more often but let more tiles share a CU, which hides the
     HBM latency of each level's face loads.  Measured on MI355X: Euler
     1024^2 b=16 with the level update (12 KB scratch) 3 levels 4.7 GCUPS,
-    4 levels 3.9; heat 8192^2 b=32 2 levels 175, 3 levels 180, 5 levels 163;
+    4 levels 3.9, and in a later build 2 levels in place 6.5 against 3
+    levels 6.0; heat 8192^2 b=32 2 levels 175, 3 levels 180, 5 levels 163;
     b=64 2 levels 245."""
     level = nv * b * b * real_bytes
     full = 2 * P + its
