@@ -75,3 +75,17 @@ def test_share_split_heat_history(device):
     split = _solve("heat", ic, g, 1, 1, 16, 0.5, history=True)
     for i in (1, 15, 30):
         assert np.array_equal(split.data[i], oracle_stages(step, ic, 1, i)), i
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("b", [8, 12, 16, 24])
+def test_reference_sweep_heat_rk2_384_at_share_half(device, b):
+    """The reference's parameter sweep (tests.py:54-77: 384^2, b in
+    {8, 12, 16, 24}, shares 0, 0.5, 1) at share 0.5 for heat RK2: the split
+    solve gives the GPU-only solve's bits, and that one the oracle's."""
+    ic, g, step = heat_case(384, False, 30, seed=5)
+    split = _solve("heat", ic, g, 1, 2, b, 0.5)
+    whole = _solve("heat", ic, g, 1, 2, b, 1.0)
+    assert split.share == 0.5
+    assert np.array_equal(split.final_state, whole.final_state)
+    assert np.array_equal(whole.final_state, oracle_stages(step, ic, 2, 60))
