@@ -81,6 +81,21 @@ int ps_memcpy_dtod(void *dst, const void *src, size_t bytes);
 /* Fill bytes (stream ordered). */
 int ps_memset(void *dst, int value, size_t bytes);
 
+/* Page-locked host memory: the host side of the asynchronous copies below
+ * (no counterpart in the reference, whose copies all wait). */
+int ps_host_malloc(void **ptr, size_t bytes);
+int ps_host_free(void *ptr);
+
+/* Copies ordered on the work stream that return without waiting for it.
+ * The host buffer must come from ps_host_malloc and stay untouched until an
+ * event recorded after the copy has completed (ps_event_query /
+ * ps_event_synchronize).  They let a serving loop fetch a finished
+ * problem's result, or stage the next one's initial condition, while the
+ * rounds queued before them still run (no counterpart in the reference). */
+int ps_memcpy_htod_async(void *dst, const void *src, size_t bytes);
+int ps_memcpy_dtoh_async(void *dst, const void *src, size_t bytes);
+int ps_memcpy_dtod_async(void *dst, const void *src, size_t bytes);
+
 /* Launch a kernel on the work stream.  args[i] points at the i-th kernel
  * argument.  Replaces the PyCUDA Function call in Geometry.callGPU /
  * callStandardGPU (core/geometry.py:38-46). */
@@ -98,6 +113,9 @@ int ps_event_elapsed_ms(ps_event start, ps_event stop, float *ms);
 /* Block the host until the work recorded before the event is done (bounds
  * how far a host loop runs ahead of the GPU). */
 int ps_event_synchronize(ps_event event);
+/* *done = 1 when the work recorded before the event has completed, else 0;
+ * never blocks. */
+int ps_event_query(ps_event event, int *done);
 int ps_event_destroy(ps_event event);
 
 /* ---------------------------------------------------------------------- */

@@ -161,7 +161,7 @@ int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xof
 
 extern "C" {
 
-int ps_version(void) { return 1; }
+int ps_version(void) { return 2; }
 
 const char *ps_last_error(void) { return g_error.c_str(); }
 
@@ -334,6 +334,40 @@ int ps_memcpy_dtod(void *dst, const void *src, size_t bytes)
     return 0;
 }
 
+int ps_host_malloc(void **ptr, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return 0;
+}
+
+int ps_host_free(void *ptr)
+{
+    if (ptr) PS_HIP(hipHostFree(ptr));
+    return 0;
+}
+
+int ps_memcpy_htod_async(void *dst, const void *src, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    return 0;
+}
+
+int ps_memcpy_dtoh_async(void *dst, const void *src, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    return 0;
+}
+
+int ps_memcpy_dtod_async(void *dst, const void *src, size_t bytes)
+{
+    PS_NEED_INIT();
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    return 0;
+}
+
 int ps_memset(void *dst, int value, size_t bytes)
 {
     PS_NEED_INIT();
@@ -382,6 +416,18 @@ int ps_event_elapsed_ms(ps_event start, ps_event stop, float *ms)
 int ps_event_synchronize(ps_event event)
 {
     PS_HIP(hipEventSynchronize(static_cast<hipEvent_t>(event)));
+    return 0;
+}
+
+int ps_event_query(ps_event event, int *done)
+{
+    const hipError_t e = hipEventQuery(static_cast<hipEvent_t>(event));
+    if (e == hipErrorNotReady) {
+        *done = 0;
+        return 0;
+    }
+    PS_HIP(e);
+    *done = 1;
     return 0;
 }
 

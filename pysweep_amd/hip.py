@@ -112,6 +112,11 @@ _htod = _sig("ps_memcpy_htod", c_void_p, c_void_p, c_size_t)
 _dtoh = _sig("ps_memcpy_dtoh", c_void_p, c_void_p, c_size_t)
 _dtod = _sig("ps_memcpy_dtod", c_void_p, c_void_p, c_size_t)
 _memset = _sig("ps_memset", c_void_p, c_int, c_size_t)
+_host_malloc = _sig("ps_host_malloc", ctypes.POINTER(c_void_p), c_size_t)
+_host_free = _sig("ps_host_free", c_void_p)
+_htod_async = _sig("ps_memcpy_htod_async", c_void_p, c_void_p, c_size_t)
+_dtoh_async = _sig("ps_memcpy_dtoh_async", c_void_p, c_void_p, c_size_t)
+_dtod_async = _sig("ps_memcpy_dtod_async", c_void_p, c_void_p, c_size_t)
 _launch = _sig("ps_launch", c_void_p, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.c_uint,
                ctypes.c_uint, ctypes.c_uint, ctypes.c_uint, ctypes.POINTER(c_void_p))
 _sync = _sig("ps_synchronize")
@@ -120,6 +125,7 @@ _event_record = _sig("ps_event_record", c_void_p)
 _event_elapsed = _sig("ps_event_elapsed_ms", c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float))
 _event_destroy = _sig("ps_event_destroy", c_void_p)
 _event_sync = _sig("ps_event_synchronize", c_void_p)
+_event_query = _sig("ps_event_query", c_void_p, ctypes.POINTER(c_int))
 _swept_run = _sig("ps_swept_run", ctypes.POINTER(SweptPlan), c_int, c_int, c_int)
 _standard_run = _sig("ps_standard_run", ctypes.POINTER(StandardPlan), c_int, c_int)
 _swept_run_timed = _sig("ps_swept_run_timed", ctypes.POINTER(SweptPlan), c_int, c_int, c_int,
@@ -139,8 +145,10 @@ _graph_destroy = _sig("ps_graph_destroy", c_void_p)
 EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shutdown", "ps_compile",
            "ps_free_host", "ps_module_load", "ps_module_unload", "ps_module_get_global",
            "ps_module_get_function", "ps_function_info", "ps_malloc", "ps_free", "ps_memcpy_htod",
-           "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_launch", "ps_synchronize",
-           "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_destroy",
+           "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_host_malloc", "ps_host_free",
+           "ps_memcpy_htod_async", "ps_memcpy_dtoh_async", "ps_memcpy_dtod_async", "ps_launch", "ps_synchronize",
+           "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_query",
+           "ps_event_destroy",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
            "ps_multi_index", "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
@@ -269,6 +277,54 @@ class DeviceArray:
             pass
 
 
+class PinnedArray:
+    """Page-locked host buffer (ps_host_malloc) with a numpy view; the host
+    side of the asynchronous copies, which return before the data moved."""
+
+    def __init__(self, nbytes):
+        ensure_init()
+        p = c_void_p()
+        check(_host_malloc(ctypes.byref(p), max(int(nbytes), 1)), "ps_host_malloc")
+        self.ptr = p.value
+        self.nbytes = int(nbytes)
+
+    def view(self, dtype, shape):
+        """numpy array over the first bytes of the buffer (valid while it lives)."""
+        dtype = numpy.dtype(dtype)
+        n = int(numpy.prod(shape))
+        assert n * dtype.itemsize <= self.nbytes
+        raw = (ctypes.c_char * (n * dtype.itemsize)).from_address(self.ptr)
+        return numpy.frombuffer(raw, dtype=dtype, count=n).reshape(shape)
+
+    def upload_async(self, dptr, nbytes, offset=0):
+        """Copy nbytes from offset of this buffer to device address dptr, in
+        stream order, without waiting."""
+        assert offset + nbytes <= self.nbytes
+        check(_htod_async(dptr, self.ptr + offset, nbytes), "ps_memcpy_htod_async")
+
+    def download_async(self, dptr, nbytes, offset=0):
+        """Copy nbytes from device address dptr to offset of this buffer, in
+        stream order, without waiting."""
+        assert offset + nbytes <= self.nbytes
+        check(_dtoh_async(self.ptr + offset, dptr, nbytes), "ps_memcpy_dtoh_async")
+
+    def close(self):
+        if self.ptr:
+            _host_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dtod_async(dst, src, nbytes):
+    """Stream-ordered device copy that returns without waiting."""
+    check(_dtod_async(dst, src, nbytes), "ps_memcpy_dtod_async")
+
+
 def htod(dptr, arr):
     a = numpy.ascontiguousarray(arr)
     check(_htod(dptr, a.ctypes.data, a.nbytes), "ps_memcpy_htod")
@@ -339,6 +395,12 @@ class Event:
 
     def synchronize(self):
         check(_event_sync(self.h), "ps_event_synchronize")
+
+    def query(self):
+        """True once the work recorded before the event has completed."""
+        done = c_int()
+        check(_event_query(self.h, ctypes.byref(done)), "ps_event_query")
+        return bool(done.value)
 
     def elapsed_ms(self, stop):
         ms = ctypes.c_float()

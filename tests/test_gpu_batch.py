@@ -90,6 +90,47 @@ def test_continuous_admission_under_hbm_budget(device):
         assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps))
 
 
+def test_reused_ring_after_async_retirement(device):
+    """One problem at a time (max_active=1): each admitted problem takes the
+    ring its predecessor just left, while that ring's final level may still
+    be on its way to the host.  Stream order must keep every result intact,
+    and the pool must serve every problem from the first ring."""
+    from pysweep_amd.batch import Engine
+    mix = [_heat_problem(64, 64, 20 + 3 * i, kx=1 + i % 3, amp=0.5 + 0.25 * i) for i in range(5)]
+    eng = Engine(max_active=1)
+    tickets = [eng.submit(p, arrival=0.0) for p, _ in mix]
+    eng.run()
+    assert [t.done for t in tickets] == [True] * 5
+    assert len(eng.rings.free) == 1 and not eng.running and eng.in_use == 0
+    for t, (p, step) in zip(tickets, mix):
+        steps = int(round((p.globals[1] - p.globals[0]) / p.globals[2]))
+        assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps)), t.problem.globals
+
+
+def test_async_copies_and_event_query(device):
+    """Page-locked buffers, the copies that do not wait, and the event
+    query that does not block."""
+    from pysweep_amd import hip
+    n = 1 << 20
+    h, back = hip.PinnedArray(8 * n), hip.PinnedArray(8 * n + 64)
+    d, d2 = hip.DeviceArray(8 * n), hip.DeviceArray(8 * n)
+    want = np.arange(n, dtype=np.float64) * 0.5 - 7.0
+    h.view(np.float64, (n,))[:] = want
+    back.view(np.uint8, (8 * n + 64,))[:] = 0xAB
+    h.upload_async(d.ptr, 8 * n)
+    hip.dtod_async(d2.ptr, d.ptr, 8 * n)
+    back.download_async(d2.ptr + 8, 8 * (n - 1), 64)  # offsets on both sides
+    ev = hip.Event()
+    ev.record()
+    ev.synchronize()
+    assert ev.query()
+    got = back.view(np.uint8, (8 * n + 64,))
+    assert (got[:64] == 0xAB).all() and (got[64 + 8 * (n - 1):] == 0xAB).all()
+    assert np.array_equal(got[64:64 + 8 * (n - 1)].view(np.float64), want[1:])
+    for a in (h, back, d, d2):
+        a.close()
+
+
 def test_block_size_choice_does_not_change_results(device):
     (p16, step), (p32, _) = _euler_problem(128, 96, 11, b=16), _euler_problem(128, 96, 11, b=32)
     (p8, _) = _euler_problem(128, 96, 11, b=8)

@@ -297,6 +297,45 @@ def test_oldest_first_selection():
     assert batch.oldest_first([], cap) == set()
 
 
+def test_buffer_pool_best_fit_and_limit():
+    """The engine's pool of freed rings / staging buffers: smallest free
+    buffer within the slack, a new one otherwise, free buffers released
+    (largest first) so that they and the new one fit the limit."""
+    from pysweep_amd import batch
+
+    class Buf:
+        def __init__(self, n):
+            self.nbytes, self.closed = n, False
+
+        def close(self):
+            self.closed = True
+
+    made = []
+
+    def make(n):
+        made.append(Buf(n))
+        return made[-1]
+
+    pool = batch._Pool(make, slack=1.25)
+    x1, x2, x3 = pool.take(100), pool.take(120), pool.take(500)
+    assert len(made) == 3 and pool.bytes == 0
+    for x in (x1, x2, x3):
+        pool.give(x)
+    assert pool.bytes == 720
+    assert pool.take(110) is x2            # 100 < 110; 120 <= 137.5
+    assert pool.take(90) is x1             # smallest >= 90 within 112.5
+    assert pool.bytes == 500
+    y = pool.take(200)                     # nothing within [200, 250]: new
+    assert y is made[-1] and y.nbytes == 200 and pool.bytes == 500
+    # a new 300 under a limit of 600 frees the 500 (free + new must fit)
+    z = pool.take(300, limit=600)
+    assert x3.closed and pool.bytes == 0 and z.nbytes == 300
+    pool.give(y)
+    pool.give(z)
+    pool.trim(250)                         # largest first: z (300) goes
+    assert z.closed and not y.closed and pool.bytes == 200
+
+
 def test_short_first_selection():
     """The engine's round selection (pysweep_amd.batch.short_first): the
     shortest remaining work first, packed up to the capacity in cells; a
@@ -330,3 +369,60 @@ def test_short_first_selection():
     lone = ticket(schedule.swept_times(1000, 16, 2, 2, exact=True), 2048)
     assert batch.short_first([lone], now) == {lone}
     assert batch.short_first([], now) == set()
+
+
+def test_staging_ring_first_in_first_out():
+    """The engine's page-locked staging ring: pieces are handed out in order,
+    wrap to the start when the end is taken, and never overlap a live piece;
+    the oldest pieces are given back first."""
+    from pysweep_amd import batch
+
+    class Buf:  # stands in for the page-locked buffer
+        def __init__(self, n):
+            self.a = np.zeros(n, np.uint8)
+            self.ptr = 0
+
+        def view(self, dtype, shape):
+            return self.a.view(dtype).reshape(shape)
+
+    A = batch._StagingRing.ALIGN
+    r = batch._StagingRing(8 * A, make=Buf)
+    assert r.alloc(9 * A) is None            # larger than the ring
+    assert r.alloc(3 * A) == 0
+    assert r.alloc(A + 1) == 3 * A           # rounded up to two units
+    assert r.alloc(3 * A) == 5 * A           # up to the end
+    assert r.alloc(1) is None                # full
+    r.release(1)                             # [0, 3A) free again
+    assert r.alloc(4 * A) is None            # does not fit before the oldest live piece
+    assert r.alloc(2 * A) == 0               # wraps
+    assert r.alloc(A) == 2 * A               # fills [2A, 3A): full again
+    assert r.alloc(1) is None
+    r.release(2)                             # [3A, 8A) free; live: [0, 3A)
+    assert r.alloc(5 * A) == 3 * A
+    r.release(3)
+    assert list(r.live) == []
+    assert r.alloc(8 * A) == 0               # empty ring: from the start
+
+    # random traffic against a byte map of the live pieces
+    rng = np.random.default_rng(3)
+    r = batch._StagingRing(64 * A, make=Buf)
+    owner = np.full(64 * A, -1)
+    live = []
+    for i in range(2000):
+        if live and (rng.uniform() < 0.45 or len(live) > 20):
+            k = int(rng.integers(1, len(live) + 1))
+            r.release(k)
+            for off, n in live[:k]:
+                owner[off:off + n] = -1
+            live = live[k:]
+            continue
+        n = int(rng.integers(1, 20 * A))
+        off = r.alloc(n)
+        if off is None:
+            continue
+        m = -(-n // A) * A
+        assert off % A == 0 and off + m <= 64 * A
+        assert (owner[off:off + m] == -1).all(), i
+        owner[off:off + m] = i
+        live.append((off, m))
+    assert [tuple(x) for x in r.live] == live
