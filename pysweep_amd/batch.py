@@ -502,10 +502,11 @@ class Engine:
 
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
-                 capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20):
+                 capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4):
         hip.ensure_init(0)
         self.policy = policy
         self.admit_bytes = admit_bytes  # initial-condition bytes staged per step at most
+        self.admit_rounds = admit_rounds  # rounds run between two such steps
         self._throttled = False
         self.capacity = capacity  # cells per round for "oldest_first" and "short_first"
         self.solo = solo  # a problem running alone uses the plain kernels
@@ -629,7 +630,7 @@ class Engine:
         retirements and arrivals)."""
         r = min(t.c_end - t.c for t in active)
         if self._throttled:  # more requests to admit after a few rounds
-            r = min(r, self.poll_rounds)
+            r = min(r, self.admit_rounds)
         elif len(active) < len(self.running) and self.policy == "short_first":
             r = min(r, self.poll_rounds)  # the selection changes with time
         elif self.queue:
