@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["swept", "standard", "both"], default="both")
     ap.add_argument("--batch", type=int, default=512, help="problems of the batched workload (0: skip)")
+    ap.add_argument("--batch-legs", default="swept,standard,plain",
+                    help="runs of the batched workload: engine with swept / standard kernels, plain one at a time")
     ap.add_argument("--serve", type=int, default=48, help="requests of the served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=4.0, help="served-stream arrival rate per second")
     ap.add_argument("--large", type=int, default=1, help="large single instances (heat 8192^2, Euler 2048^2)")
@@ -164,16 +166,19 @@ def cpu_baseline(ic, g, stages, workers=None):
             "np_version": np.__version__}
 
 
-def batched_workload(n, seed=0):
+def batched_workload(n, seed=0, legs=("swept", "standard", "plain")):
     """Independent heat and Euler problems, 200 steps each, FP64: the batched
     engine (instance-table launches, graph-replayed rounds) against solving
-    them one at a time with the plain standard kernel."""
+    them one at a time with the plain standard kernel.  ``legs`` narrows the
+    study to some of the three runs (a profile of one engine run uses it)."""
     from pysweep_amd import baseline, batch, hip, workloads
     probs = workloads.batched_mix(n=n, steps=200, seed=seed)
     cells = sum(p.ic.shape[1] * p.ic.shape[2] * 200 for p in probs)
     out = {"problems": n, "steps": 200, "cell_steps": cells}
     results = {}
     for label, method in (("engine_swept", "swept"), ("engine_standard", "standard")):
+        if method not in legs:
+            continue
         for p in probs:
             p.method = method
         eng = batch.Engine()
@@ -187,16 +192,19 @@ def batched_workload(n, seed=0):
         results[label] = {t.problem.tag: t.result for t in eng.finished}
         out[label] = {"seconds": sec, "solves_per_s": n / sec, "gcups": cells / sec / 1e9, "rounds": eng.rounds,
                       "peak_hbm_gb": eng.peak_in_use / 1e9}
+    if "plain" not in legs:
+        return out
     t0 = time.perf_counter()
     same = 0
     for p in probs:
         r = baseline.solve_plain(p)
-        same += int(numpy_equal(r, results["engine_swept"][p.tag]) and numpy_equal(r, results["engine_standard"][p.tag]))
+        same += int(all(numpy_equal(r, res[p.tag]) for res in results.values()))
     sec = time.perf_counter() - t0
     out["one_at_a_time_plain"] = {"seconds": sec, "solves_per_s": n / sec, "gcups": cells / sec / 1e9}
     out["bitwise_equal_results"] = same
-    best = max(out["engine_swept"]["solves_per_s"], out["engine_standard"]["solves_per_s"])
-    out["engine_over_plain"] = best / out["one_at_a_time_plain"]["solves_per_s"]
+    if results:
+        best = max(out[k]["solves_per_s"] for k in results)
+        out["engine_over_plain"] = best / out["one_at_a_time_plain"]["solves_per_s"]
     return out
 
 
@@ -516,7 +524,7 @@ def main():
     if not a.no_cpu:
         out["heat_cpu_case"] = heat_cpu_case()
     if a.batch:
-        out["batched"] = batched_workload(a.batch)
+        out["batched"] = batched_workload(a.batch, legs=tuple(a.batch_legs.split(",")))
     if a.serve:
         out["serving"] = served_workload(a.serve, a.rate)
         out["p50_latency_s"] = out["serving"]["engine"]["p50_s"]
