@@ -217,7 +217,6 @@ typedef struct ps_inst {
     int c;               /* round counter: 0 FirstPrism, 1..G UpPrism,        */
                          /* G+1 LastPrism, G+2.. tail stages                 */
     int active;
-    int tpi;             /* standard tiles per work item (>= 1)           */
 } ps_inst;
 
 typedef struct ps_multi_plan {
@@ -227,8 +226,8 @@ typedef struct ps_multi_plan {
     void *owner;        /* device int[total_tiles]: problem of each tile    */
     void *sowner;       /* device int[total_stiles]                         */
     int ninst;
-    int total_tiles, total_stiles;
-    int grid, sgrid;    /* persistent workgroups; 0 = one per tile          */
+    int total_tiles, total_stiles;  /* work items: one b x b swept tile, or one
+                                       standard tile, each; one workgroup each */
     int threads, sthreads;
     ps_function up, ybridge, xbridge, octahedron, down, standard, advance;
 } ps_multi_plan;

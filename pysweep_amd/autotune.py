@@ -21,16 +21,15 @@ DEFAULT_PATH = os.path.join(compiler.CACHE, "tuning.json")
 #: 178 GCUPS with b=32 and 271 with b=64, which only grids that are
 #: multiples of 64 can use.
 DEFAULTS = {
-    ("euler", "fp64"): {"b": 16, "grid_per_cu": 0},
-    ("euler", "fp32"): {"b": 16, "grid_per_cu": 0},
-    ("heat", "fp64"): {"b": 32, "grid_per_cu": 0},
-    ("heat", "fp32"): {"b": 32, "grid_per_cu": 0},
+    ("euler", "fp64"): {"b": 16},
+    ("euler", "fp32"): {"b": 16},
+    ("heat", "fp64"): {"b": 32},
+    ("heat", "fp32"): {"b": 32},
 }
 
 CANDIDATES = {
-    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "grid_per_cu": 8}, {"b": 32, "grid_per_cu": 4},
-              {"b": 16, "method": "standard"}],
-    "heat": [{"b": 16}, {"b": 32}, {"b": 32, "grid_per_cu": 8}, {"b": 64}, {"b": 32, "method": "standard"}],
+    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "method": "standard"}],
+    "heat": [{"b": 16}, {"b": 32}, {"b": 64}, {"b": 32, "method": "standard"}],
 }
 
 
@@ -49,7 +48,7 @@ class Tuner:
     def record(self, plugin, precision, its):
         r = self.records.get(_key(plugin, precision, its))
         if r is None:
-            r = dict(DEFAULTS.get((plugin, precision), {"b": 16, "grid_per_cu": 0}))
+            r = dict(DEFAULTS.get((plugin, precision), {"b": 16}))
         return r
 
     def block_size(self, plugin, precision, its, nx, ny):
@@ -67,7 +66,7 @@ class Tuner:
 
     def config(self, plugin, precision, its, b):
         r = self.record(plugin, precision, its)
-        out = {"grid_per_cu": r.get("grid_per_cu", 0)}
+        out = {}
         if r.get("b") == b:
             for k in ("slots", "threads"):
                 if k in r:
@@ -110,7 +109,7 @@ class Tuner:
                            "results": {t.problem.tag: t.result for t in eng.finished}})
         best = max(trials, key=lambda r: r["gcups"])
         self.last_trials = trials
-        rec = {k: v for k, v in best.items() if k in ("b", "slots", "threads", "grid_per_cu", "method")}
+        rec = {k: v for k, v in best.items() if k in ("b", "slots", "threads", "method")}
         trials = [{k: v for k, v in t.items() if k != "results"} for t in trials]
         rec["trials"] = trials
         self.records[_key(plugin, precision, its)] = rec

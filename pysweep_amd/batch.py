@@ -42,16 +42,6 @@ PLUGINS = {
 }
 
 
-def tiles_per_item(tiles, slots=2048, most=4):
-    """Consecutive standard tiles per work item of a batched standard
-    launch: a work item's table lookups and constant staging (three
-    dependent loads) are paid once per item -- at one tile per item they
-    cost a solo 2048^2 Euler solve 19 % against the plain kernel -- but a
-    problem needs enough items to fill the GPU (``slots`` ~ 2 workgroups per
-    SIMD slot of 256 CUs), so only large problems take up to ``most``."""
-    return max(1, min(most, tiles // slots))
-
-
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
 #: side (Euler FP64 1024^2: standard 7.1 GCUPS, swept b=16 4.7; heat 8192^2
@@ -137,7 +127,7 @@ def shipped_class_specs():
 class PluginClass:
     """Compiled instance-table modules of one (plugin, precision, b)."""
 
-    def __init__(self, plugin, precision, its, b, slots=None, threads=None, grid_per_cu=0):
+    def __init__(self, plugin, precision, its, b, slots=None, threads=None):
         py, cu64, cu32, nv, ops, default_its = PLUGINS[plugin]
         self.plugin, self.precision, self.nv, self.ops, self.its, self.b = plugin, precision, nv, ops, its, b
         self.real = "double" if precision == "fp64" else "float"
@@ -152,7 +142,6 @@ class PluginClass:
         self.fn["Standard"] = sm.get_function("StandardMulti")
         self.fn["advance"] = m.get_function("ps_advance")
         self.cpu = plugins.load_cpu_module(eq_path(py), name=f"pysweep_batch_{plugin}")
-        self.grid_per_cu = grid_per_cu
         self.key = (plugin, precision, its, b)
         self._solo = None
         self.solo_consts = None  # problem whose constants the solo modules hold
@@ -404,13 +393,11 @@ class _ClassTable:
             st.stiles_x, st.stiles_y = -(-t.nx // sbx), -(-t.ny // sby)
             st.G, st.tail, st.c = t.times.G, t.times.tail, t.c
             st.active = int(active is None or t in active)
-            st.tpi = tiles_per_item(st.stiles_x * st.stiles_y)
             # problems without swept cycles (G = -1) take no swept work items,
             # problems sitting out of the rounds none at all
             run = bool(st.active)
             pre[i + 1] = pre[i] + (st.tiles_x * st.tiles_y if t.times.G >= 0 and run else 0)
-            items = -(-st.stiles_x * st.stiles_y // st.tpi)
-            spre[i + 1] = spre[i] + (items if t.times.tail and run else 0)
+            spre[i + 1] = spre[i] + (st.stiles_x * st.stiles_y if t.times.tail and run else 0)
         if host is not None:
             host.upload_async(self.insts.ptr, isz)
             host.upload_async(self.prefix.ptr, psz, isz)
@@ -428,12 +415,11 @@ class _ClassTable:
                     buf.close()
                 setattr(self, attr, hip.DeviceArray(8 * total))
         f = cls.fn
-        grid = cls.grid_per_cu * cus if cls.grid_per_cu else 0
         self.plan = hip.MultiPlan(insts=self.insts.ptr, prefix=self.prefix.ptr, sprefix=self.sprefix.ptr,
                                   owner=self.owner.ptr if getattr(self, "owner", None) else None,
                                   sowner=self.sowner.ptr if getattr(self, "sowner", None) else None,
-                                  ninst=n, total_tiles=int(pre[-1]), total_stiles=int(spre[-1]), grid=grid,
-                                  sgrid=grid, threads=cls.spec.threads, sthreads=cls.sspec.threads,
+                                  ninst=n, total_tiles=int(pre[-1]), total_stiles=int(spre[-1]),
+                                  threads=cls.spec.threads, sthreads=cls.sspec.threads,
                                   up=f["UpPyramid"], ybridge=f["YBridge"], xbridge=f["XBridge"],
                                   octahedron=f["Octahedron"], down=f["DownPyramid"], standard=f["Standard"],
                                   advance=f["advance"])
@@ -755,8 +741,7 @@ class Engine:
         cls = self.classes.get(key)
         if cls is None:
             cfg = self.tuner.config(p.plugin, p.precision, p.its, b) if self.tuner else {}
-            cls = PluginClass(p.plugin, p.precision, p.its, b, slots=cfg.get("slots"), threads=cfg.get("threads"),
-                              grid_per_cu=cfg.get("grid_per_cu", 0))
+            cls = PluginClass(p.plugin, p.precision, p.its, b, slots=cfg.get("slots"), threads=cfg.get("threads"))
             self.classes[key] = cls
             self.tables[key] = _ClassTable(cls)
         return cls

@@ -560,7 +560,7 @@ static int multi_round(const ps_multi_plan *p)
     void *sargs[] = {&insts, &sprefix, &sowner, &n};
     int rc = 0;
     if (p->total_tiles > 0) {
-        const unsigned g = static_cast<unsigned>(p->grid > 0 && p->grid < p->total_tiles ? p->grid : p->total_tiles);
+        const unsigned g = static_cast<unsigned>(p->total_tiles);  // one workgroup per work item
         const unsigned t = static_cast<unsigned>(p->threads);
         // Within a round every problem keeps its own phase order:
         // FirstPrism Up<Yb, UpPrism Xb<Oct<Yb, LastPrism Xb<Down.
@@ -571,8 +571,7 @@ static int multi_round(const ps_multi_plan *p)
         if ((rc = launch_kind(p->ybridge, g, t, args, PS_K_YBRIDGE))) return rc;
     }
     if (p->total_stiles > 0 && p->standard) {
-        const unsigned g =
-            static_cast<unsigned>(p->sgrid > 0 && p->sgrid < p->total_stiles ? p->sgrid : p->total_stiles);
+        const unsigned g = static_cast<unsigned>(p->total_stiles);
         if ((rc = launch_kind(p->standard, g, static_cast<unsigned>(p->sthreads), sargs, PS_K_STANDARD))) return rc;
     }
     void *aargs[] = {&insts, &n};

@@ -49,13 +49,13 @@ class Inst(ctypes.Structure):
     """ps_inst: one problem of an instance-table launch."""
     _fields_ = [("ring", c_void_p), ("consts", c_void_p), ("nx", c_int), ("ny", c_int), ("T", c_int),
                 ("tiles_x", c_int), ("tiles_y", c_int), ("stiles_x", c_int), ("stiles_y", c_int),
-                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int), ("tpi", c_int)]
+                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int)]
 
 
 class MultiPlan(ctypes.Structure):
     _fields_ = [("insts", c_void_p), ("prefix", c_void_p), ("sprefix", c_void_p), ("owner", c_void_p),
                 ("sowner", c_void_p), ("ninst", c_int),
-                ("total_tiles", c_int), ("total_stiles", c_int), ("grid", c_int), ("sgrid", c_int),
+                ("total_tiles", c_int), ("total_stiles", c_int),
                 ("threads", c_int), ("sthreads", c_int),
                 ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p), ("octahedron", c_void_p),
                 ("down", c_void_p), ("standard", c_void_p), ("advance", c_void_p)]

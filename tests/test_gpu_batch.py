@@ -197,7 +197,7 @@ def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(dev
             out.append(p)
         return out
 
-    cands = [{"b": 16}, {"b": 32}, {"b": 32, "grid_per_cu": 2}, {"b": 32, "method": "standard"}]
+    cands = [{"b": 16}, {"b": 32}, {"b": 32, "slots": 4}, {"b": 32, "method": "standard"}]
     rec = tu.tune("heat", "fp64", 1, make, candidates=cands)
     assert rec["b"] in (16, 32) and rec["method"] in ("swept", "standard")
     with open(path) as f:
