@@ -281,12 +281,14 @@ _FREE_RINGS = []  # staging rings of finished engines, for the next engine
 STAGING_BYTES = 1 << 30
 
 
-def _take_ring(nbytes=STAGING_BYTES):
+def _take_ring(nbytes=0):
+    """A free staging ring of at least ``nbytes`` (and STAGING_BYTES)."""
+    nbytes = max(nbytes, STAGING_BYTES)
     for r in sorted(_FREE_RINGS, key=lambda r: r.size):
         if r.size >= nbytes:
             _FREE_RINGS.remove(r)
             return r
-    return _StagingRing(max(nbytes, STAGING_BYTES))
+    return _StagingRing(nbytes)
 
 
 @dataclasses.dataclass(eq=False)

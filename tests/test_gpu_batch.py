@@ -107,6 +107,26 @@ def test_reused_ring_after_async_retirement(device):
         assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps)), t.problem.globals
 
 
+def test_small_staging_ring_wraps_waits_and_grows(device, monkeypatch):
+    """A staging ring far smaller than the traffic: admissions and
+    retirements wait for earlier copies to free room, pieces wrap around
+    the ring, and a level larger than the ring makes a bigger one; every
+    result must still equal the oracle."""
+    from pysweep_amd import batch
+    monkeypatch.setattr(batch, "STAGING_BYTES", 1 << 16)  # 64 KiB
+    monkeypatch.setattr(batch, "_FREE_RINGS", [])
+    mix = _mix()  # levels of 32 KiB .. 512 KiB
+    eng = batch.Engine(admit_bytes=1 << 16)
+    assert eng._ring.size == 1 << 16
+    tickets = [eng.submit(p, arrival=0.0) for p, _ in mix]
+    eng.run()
+    assert all(t.done for t in tickets)
+    assert max(r.size for r in batch._FREE_RINGS) > 1 << 16  # grew for the largest level
+    for t, (p, step) in zip(tickets, mix):
+        steps = int(round((p.globals[1] - p.globals[0]) / p.globals[2]))
+        assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps))
+
+
 def test_async_copies_and_event_query(device):
     """Page-locked buffers, the copies that do not wait, and the event
     query that does not block."""
