@@ -27,7 +27,6 @@ import collections
 import ctypes
 import dataclasses
 import math
-import os
 import time
 
 import numpy
@@ -355,7 +354,7 @@ class _ClassTable:
         """True when the table's run flags differ from ``active``."""
         return self.flags != tuple(t in active for t in self.running)
 
-    def rebuild(self, cus, active=None, stage=None):
+    def rebuild(self, active=None, stage=None):
         """Write the table of the running problems (those in ``active``, or
         all, take part in the rounds).  ``stage(nbytes)`` returns a page-locked
         buffer that stays untouched until the copies queued now have run; the
@@ -488,7 +487,7 @@ class Engine:
     (17.05-17.12 solves/s against 16.97-17.10; 512-problem mix 193-195
     against 191-195)."""
 
-    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, cus=256, tuner=None, clock=None,
+    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, tuner=None, clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
                  capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4):
         hip.ensure_init(0)
@@ -510,7 +509,6 @@ class Engine:
         self.graph_rounds = graph_rounds
         self.poll_rounds = poll_rounds
         self.max_rounds_ahead = max_rounds_ahead
-        self.cus = cus
         self.tuner = tuner
         self.classes = {}
         self.tables = {}
@@ -707,7 +705,7 @@ class Engine:
         R = self._rounds_ahead(active)
         for key, tab in self.tables.items():
             if tab.dirty or tab.needs(active):
-                tab.rebuild(self.cus, active, stage=self._stage)
+                tab.rebuild(active, stage=self._stage)
             if tab.plan is None or not any(tab.flags):
                 continue
             K = self.graph_rounds
