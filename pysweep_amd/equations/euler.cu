@@ -41,6 +41,18 @@ __device__ __forceinline__ eu_q eu_at(const R *s, int i)
     return q;
 }
 
+/* the same from an object indexed like a level slot (s[i]) */
+template <class S>
+__device__ __forceinline__ eu_q eu_at(const S &s, int i)
+{
+    eu_q q;
+    q.r = s[i];
+    q.m = s[i + VARS];
+    q.n = s[i + 2 * VARS];
+    q.e = s[i + 3 * VARS];
+    return q;
+}
+
 /*
  * Division.  n/d is formed as the gfx950 compiler's IEEE double division
  * forms it -- reciprocal estimate, two Newton-Raphson refinements, product,
@@ -254,10 +266,15 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
  * bitwise those of step().
  *
  *   cur    level r-1 in LDS, variable v of cell (x, y) at
- *          cur[v*VARS + x*SY + y]; prev is level r-2 (the base of a final
- *          stage) and out receives level r, with the same strides (slots
- *          passed separately, so the caller may keep its window circular;
- *          out may equal prev);
+ *          cur[v*VARS + x*SY + y];
+ *   prev   level r-2 (the base of a final stage), indexed like cur
+ *          (prev[v*VARS + x*SY + y]) and read only at the cells of S, by
+ *          the lane computing the cell, before that lane writes it: an LDS
+ *          slot in the swept kernels, an accessor of the ring in HBM in the
+ *          standard kernel;
+ *   out    level r in LDS with cur's strides (a slot of its own, so the
+ *          caller may keep its window circular; it may be prev's slot), or
+ *          null when only emit needs the values;
  *   sc     scratch of PS_LEVEL_SCRATCH(SX * SY, PS_MAXFACES) doubles,
  *          PS_MAXFACES being the most faces of one direction of any
  *          rectangle the harness passes ((x1-x0+1)*(y1-y0) or
@@ -347,8 +364,8 @@ __device__ __forceinline__ eu_q eu_face_diff(const double *fa, const double *fb)
     return a;
 }
 
-template <class R, int MAXC, class Emit>
-__device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
+template <class R, int MAXC, class Prev, class Emit>
+__device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out, int x0, int x1, int y0, int y1,
                                          int globalTimeStep, double *sc, const Emit &emit)
 {
     const int h = x1 - x0, w = y1 - y0;
@@ -424,7 +441,7 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
             const double *fs = Fy + (i * wf + j);
             const eu_q a = eu_face_diff(fs, fs + 1);
             const int idx = (x0 + i) * SY + (y0 + j);
-            const eu_q b = eu_at(fin ? prev : cur, idx);
+            const eu_q b = fin ? eu_at(prev, idx) : eu_at(cur, idx);
             R v[4];
             if (fin) {
                 v[0] = (R)(b.r + (fx.r * DTDX + a.r * DTDY));
@@ -437,11 +454,13 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
                 v[2] = (R)(b.n + hh * (fx.n * DTDX + a.n * DTDY));
                 v[3] = (R)(b.e + hh * (fx.e * DTDX + a.e * DTDY));
             }
-            R *o = out + idx;
-            o[0] = v[0];
-            o[VARS] = v[1];
-            o[2 * VARS] = v[2];
-            o[3 * VARS] = v[3];
+            if (out) {
+                R *o = out + idx;
+                o[0] = v[0];
+                o[VARS] = v[1];
+                o[2 * VARS] = v[2];
+                o[3 * VARS] = v[3];
+            }
             emit(x0 + i, y0 + j, v);
         }
     }

@@ -50,11 +50,13 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
 /*
  * Level update used by the pysweep_amd kernels (PS_HAS_LEVEL): the cells of
  * [x0,x1) x [y0,y1) of one level, lane t taking cells t, t + PS_NT, ...
- * (row-major, y fastest).  cur is level r-1, prev level r-2 (the base of a
- * final RK2 stage), out level r, all with the tile strides; the arithmetic is
- * heat_step's, so results are bitwise the same.  Separate level pointers let
- * the swept kernels keep their LDS window circular (no copies).  Each new
- * value is also handed to emit(x, y, values), which writes it to HBM.
+ * (row-major, y fastest).  cur is level r-1 with the tile strides, prev
+ * level r-2 (the base of a final RK2 stage) indexed the same way (an LDS
+ * slot, or an accessor of the ring), out level r with the tile strides (or
+ * null); the arithmetic is heat_step's, so
+ * results are bitwise the same.  Separate levels let the swept kernels keep
+ * their LDS window circular (no copies).  Each new value is also handed to
+ * emit(x, y, values), which writes it to HBM.
  */
 #define PS_HAS_LEVEL 1
 #define PS_LEVEL_SCRATCH(CELLS, FACES) 1
@@ -77,8 +79,8 @@ __device__ __forceinline__ int heat_div(int t, int w, float rw)
     return q;
 }
 
-template <class R, int MAXC, class Emit>
-__device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, int x0, int x1, int y0, int y1,
+template <class R, int MAXC, class Prev, class Emit>
+__device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out, int x0, int x1, int y0, int y1,
                                          int globalTimeStep, double *, const Emit &emit)
 {
     const int w = y1 - y0, n = (x1 - x0) * w;
@@ -86,7 +88,8 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
     const bool fin = globalTimeStep % 2 == 0;
     for (int t = threadIdx.x; t < n; t += PS_NT) {
         const int i = heat_div(t, w, rw);
-        const int idx = (x0 + i) * SY + y0 + (t - i * w);
+        const int j = t - i * w;
+        const int idx = (x0 + i) * SY + y0 + j;
         const double d = heat_laplace(cur, idx);
         R v[1];
         if (SCHEME)
@@ -95,7 +98,8 @@ __device__ __forceinline__ void ps_level(const R *cur, const R *prev, R *out, in
             v[0] = (R)(1.0 * d + (double)prev[idx]);
         else
             v[0] = (R)(0.5 * d + (double)cur[idx]);
-        out[idx] = v[0];
-        emit(x0 + i, y0 + (t - i * w), v);
+        if (out)
+            out[idx] = v[0];
+        emit(x0 + i, y0 + j, v);
     }
 }

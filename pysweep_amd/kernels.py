@@ -149,10 +149,13 @@ def min_slots(user_path, its, generic=False):
 
 
 def standard_slots(user_path, its, generic=False):
-    """LDS level slots of the standard kernel: the ITS input levels and the
-    output, which an in-place level update writes over level A-1 (ITS >= 2)."""
-    inplace = level_update_in_standard(user_path) and level_in_place(user_path, generic) and its >= 2
-    return its if inplace else its + 1
+    """LDS level slots of the standard kernel: with a plugin level update,
+    level A alone (the update reads level A-1 from the ring at each lane's
+    own cells and hands level A+1 to the kernel's HBM stores); with the
+    per-cell step, the ITS input levels and the output."""
+    if not generic and has_level_update(user_path) and level_update_in_standard(user_path):
+        return 1
+    return its + 1
 
 
 def level_scratch_bytes(user_path, generic, sx, sy, faces, kind="swept"):
