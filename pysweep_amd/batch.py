@@ -43,8 +43,8 @@ PLUGINS = {
 
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
-#: side (Euler FP64 1024^2: standard 7.1 GCUPS, swept b=16 4.7; heat 8192^2
-#: b=32: standard 217, swept 180 -- swept wins only from b=64 on, which
+#: side (Euler FP64 1024^2: standard 11.9 GCUPS, swept b=16 9.2; heat 8192^2:
+#: standard 222, swept b=32 183 -- swept wins only from b=64 on (290), which
 #: these grid sizes rarely allow).
 DEFAULT_METHOD = {
     ("euler", "fp64"): "standard",
