@@ -16,9 +16,9 @@ from pysweep_amd import compiler
 DEFAULT_PATH = os.path.join(compiler.CACHE, "tuning.json")
 
 #: defaults from MI355X measurements of single-problem swept launches: the
-#: Euler vortex at 2048^2 ran 7.6 GCUPS with b=16 against 6.6 with b=32 (the
+#: Euler vortex at 2048^2 ran 9.9 GCUPS with b=16 against 8.8 with b=32 (the
 #: b=32 tile's 126 KB of LDS leaves one workgroup per CU); heat 8192^2 ran
-#: 178 GCUPS with b=32 and 271 with b=64, which only grids that are
+#: 183 GCUPS with b=32 and 287 with b=64, which only grids that are
 #: multiples of 64 can use.
 DEFAULTS = {
     ("euler", "fp64"): {"b": 16},

@@ -43,9 +43,8 @@ PLUGINS = {
 
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
-#: side (Euler FP64 1024^2: standard 11.9 GCUPS, swept b=16 9.2; heat 8192^2:
-#: standard 222, swept b=32 183 -- swept wins only from b=64 on (290), which
-#: these grid sizes rarely allow).
+#: side (Euler FP64 1024^2: standard 12.4 GCUPS, swept b=16 9.5; heat 8192^2:
+#: standard 299, swept b=32 183, b=64 287).
 DEFAULT_METHOD = {
     ("euler", "fp64"): "standard",
     ("euler", "fp32"): "standard",

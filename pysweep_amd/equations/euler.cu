@@ -298,9 +298,13 @@ __device__ __forceinline__ float eu_rcpf(int w) { return __builtin_amdgcn_rcpf((
 
 /* t / w for 0 <= t < 2^22, w >= 1, rw within one ulp of 1/w (the hardware
  * reciprocal, eu_rcpf): the float quotient is within one of the true one;
- * one correction makes it exact */
+ * one correction makes it exact.  A width known at compile time (the
+ * standard tile, unrolled swept levels) divides by the constant instead
+ * (measured: Euler 1024^2 Octahedron 88.2 -> 86.0 us, Standard -3 %). */
 __device__ __forceinline__ int eu_div(int t, int w, float rw)
 {
+    if (__builtin_constant_p(w))
+        return t / w;
     int q = (int)((float)t * rw);
     const int r = t - q * w;
     if (r < 0)

@@ -62,9 +62,6 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
 #define PS_LEVEL_SCRATCH(CELLS, FACES) 1
 /* prev is read only at the cell being written, so out may be prev's slot */
 #define PS_LEVEL_INPLACE 1
-/* the standard kernel keeps the per-cell step (one pass over the tile,
- * measured faster there) */
-#define PS_LEVEL_SWEPT_ONLY 1
 
 /* t / w for 0 <= t < 2^22, w >= 1, rw within one ulp of 1/w (the hardware
  * reciprocal): the float quotient is within one of t / w; one correction */
@@ -87,7 +84,9 @@ __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out,
     const float rw = __builtin_amdgcn_rcpf((float)w);
     const bool fin = globalTimeStep % 2 == 0;
     for (int t = threadIdx.x; t < n; t += PS_NT) {
-        const int i = heat_div(t, w, rw);
+        // a compile-time width (the standard tile, unrolled swept levels)
+        // divides exactly by the constant; otherwise the float reciprocal
+        const int i = __builtin_constant_p(w) ? t / w : heat_div(t, w, rw);
         const int j = t - i * w;
         const int idx = (x0 + i) * SY + y0 + j;
         const double d = heat_laplace(cur, idx);
