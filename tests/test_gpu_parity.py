@@ -239,6 +239,27 @@ def test_standard_ragged_tiles(device):
     assert np.array_equal(sol.final_state, want)
 
 
+@pytest.mark.parametrize("fe", [True, False])
+def test_standard_ragged_tiles_heat(device, fe):
+    """Heat on the standard kernel's 32x64 tiles with both sides ragged
+    (40 x 72): the level update reads level A-1 from the ring, and the
+    cells of the partial tiles beyond the grid must neither be read nor
+    stored; forward Euler and RK2 (which reads level A-1)."""
+    nx, ny, steps = 40, 72, 7
+    dx = 1.0 / nx
+    dt = 0.1 * dx * dx
+    x = np.linspace(0, 1, nx, endpoint=False)
+    y = np.linspace(0, 1, ny, endpoint=False)
+    ic = (1.3 * np.outer(np.sin(2 * np.pi * x), np.sin(4 * np.pi * y)))[None]
+    its = 1 if fe else 2
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, fe]
+    c = o_heat.courant_py(1.0, dt, dx)
+    sol = gpu_solve("heat", ic, g, 1, its, 8, swept=False, exact=True)
+    want = oracle_stages(lambda q, qb, gg: o_heat.step_py(q[0], None if qb is None else qb[0], gg, c, fe)[None],
+                         ic, its, its * steps)
+    assert np.array_equal(sol.final_state, want)
+
+
 @pytest.mark.parametrize("b", [32, 64])
 def test_large_heat_swept_equals_standard_and_decays(device, b):
     """Heat 8192^2 (the large-instance size), 200 forward-Euler steps:
