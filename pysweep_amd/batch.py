@@ -93,10 +93,7 @@ def class_specs(plugin, precision, its, b, slots=None, threads=None):
     cu = eq_path(cu64 if precision == "fp64" else cu32)
     P = b // (2 * ops) - 1
     spec = kernels.swept_spec(cu, nv, ops, its, b, slots=slots, threads=threads, real=real, multi=True)
-    bx, by = kernels.standard_tile(nv, ops, its, b * 4, b * 4, real_bytes=rb,
-                                   scratch=kernels.level_scratch(cu) if kernels.level_update_in_standard(cu) else None)
-    sspec = kernels.standard_spec(cu, nv, ops, its, max(bx, b), max(by, b), bx=bx, by=by, real=real, multi=True,
-                                  mpss=P)
+    sspec = kernels.standard_spec(cu, nv, ops, its, b * 4, b * 4, real=real, multi=True, mpss=P)
     return spec, sspec
 
 

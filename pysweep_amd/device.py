@@ -64,7 +64,7 @@ class _Run:
 class StandardRun(_Run):
     """Standard decomposition: one launch of ``Standard`` per stage."""
 
-    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, T=None, real="double",
+    def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=None, T=None, real="double",
                  generic=False):
         super().__init__(nv, its, nx, ny, T or its + 1, real)
         self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads, real=real,

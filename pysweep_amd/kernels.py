@@ -223,8 +223,11 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scr
     the faces), every pass of the update (the cells' pressures on the
     cross-shaped footprint, the x faces, the y faces) must fit in one sweep
     of the workgroup's lanes: a second, mostly idle sweep costs as much as
-    the first.  Measured on MI355X (Euler 1024^2, 256 lanes): 8x16 7.1
-    GCUPS, 8x32 6.6, 16x16 6.5."""
+    the first, and among tiles of as many cells the one with the least halo
+    wins over rows of 32 values.  Measured on MI355X (Euler 1024^2, level A
+    alone in LDS): 16x16 on 512 lanes 13.6 GCUPS, 8x32 on 512 lanes 13.4,
+    8x16 on 256 lanes 12.8, 16x8 on 256 lanes 12.7 (with levels A-1 and A
+    in LDS, 256 lanes: 8x16 7.1, 8x32 6.6, 16x16 6.5)."""
     faces_shared = scratch is not None and scratch(0, 1) > scratch(0, 0)
     slots = its + 1 if slots is None else slots
     best = None
@@ -243,7 +246,7 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scr
             if lds > lds_budget:
                 continue
             halo = sx * sy / (bx * by)
-            score = (bx * by, by >= 32, -halo)
+            score = (bx * by, -halo, by >= 32) if faces_shared else (bx * by, by >= 32, -halo)
             if best is None or score > best[0]:
                 best = (score, bx, by)
     if best is None:
@@ -251,10 +254,14 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scr
     return best[1], best[2]
 
 
-def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=256, lds_budget=LDS_BUDGET,
+def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=None, lds_budget=LDS_BUDGET,
                   real="double", multi=False, mpss=1, generic=False):
+    sc = None if generic or not level_update_in_standard(user_path) else level_scratch(user_path)
+    if threads is None:
+        # a level update that shares faces through LDS gets 512 lanes, so
+        # that a 16x16 tile's passes fit one sweep (measured in standard_tile)
+        threads = 512 if sc is not None and sc(0, 1) > sc(0, 0) else 256
     if bx is None or by is None:
-        sc = None if generic or not level_update_in_standard(user_path) else level_scratch(user_path)
         bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, sc, threads,
                                standard_slots(user_path, its, generic))
     if bx > nx or by > ny:

@@ -230,7 +230,7 @@ class Solver(object):
                                         tail_standard=True, generic=t.get("generic", False))
         else:
             self._run = device.StandardRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
-                                           bx=t.get("bx"), by=t.get("by"), threads=t.get("threads", 256),
+                                           bx=t.get("bx"), by=t.get("by"), threads=t.get("threads"),
                                            generic=t.get("generic", False))
         # CPU globals, then the plugin's device constants (core/block.py:50, 90)
         self.cpu.set_globals(*self.globals)
