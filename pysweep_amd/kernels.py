@@ -193,7 +193,16 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
     scratch = level_scratch_bytes(user_path, generic, b, b + pad, swept_faces(b, ops))
     least = min_slots(user_path, its, generic)
     if slots is None:
-        slots = default_slots(nv, its, b, P, rb, scratch, least)
+        sc = None if generic else level_scratch(user_path)
+        if sc is not None and sc(0, 1) > sc(0, 0) and level_in_place(user_path, generic):
+            # a level update that shares faces through LDS is bound by VALU
+            # issue, not by the face loads a longer window would hide, and
+            # each level it keeps costs a workgroup's LDS: the fewest levels
+            # (Euler 2048^2 b=16 with FP32 state: 2 levels 11.9 GCUPS, 4
+            # levels 11.0)
+            slots = least
+        else:
+            slots = default_slots(nv, its, b, P, rb, scratch, least)
     slots = min(slots, full)
     if slots < least:
         raise ValueError(f"an LDS window needs at least {least} levels here "
