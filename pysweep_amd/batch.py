@@ -97,13 +97,31 @@ def class_specs(plugin, precision, its, b, slots=None, threads=None):
     return spec, sspec
 
 
-def solo_specs(plugin, precision, its, b):
-    """The plain modules a class uses for a problem running alone."""
+def solo_standard_spec(plugin, precision, its, nx, ny):
+    """The plain standard module a problem of this grid uses when it runs
+    alone (its tile fits the grid)."""
     py, cu64, cu32, nv, ops, _ = PLUGINS[plugin]
     real = "double" if precision == "fp64" else "float"
     cu = eq_path(cu64 if precision == "fp64" else cu32)
-    return [kernels.swept_spec(cu, nv, ops, its, b, real=real),
-            kernels.standard_spec(cu, nv, ops, its, 1 << 16, 1 << 16, real=real)]
+    return kernels.standard_spec(cu, nv, ops, its, nx, ny, real=real)
+
+
+def solo_specs(plugin, precision, its, b):
+    """The plain modules a class uses for a problem running alone: the
+    swept phases and the standard modules of the tiles its grids can take
+    (a tile no larger than the grid; the tile depends on the grid only
+    through that bound)."""
+    py, cu64, cu32, nv, ops, _ = PLUGINS[plugin]
+    real = "double" if precision == "fp64" else "float"
+    cu = eq_path(cu64 if precision == "fp64" else cu32)
+    out = [kernels.swept_spec(cu, nv, ops, its, b, real=real)]
+    sides = [s for s in (16, 32, 64, 128, 1 << 16) if s >= b]
+    for nx in sides:
+        for ny in sides:
+            spec = solo_standard_spec(plugin, precision, its, nx, ny)
+            if spec not in out:
+                out.append(spec)
+    return out
 
 
 def shipped_class_specs():
@@ -144,15 +162,18 @@ class PluginClass:
     def ring_bytes(self, nx, ny):
         return (2 * self.P + self.its) * self.nv * nx * ny * self.rb
 
-    def solo_modules(self):
+    def solo_modules(self, nx, ny):
         """The class's one-problem modules (the plain swept phases and the
-        standard kernel), for a problem that runs alone."""
+        standard kernel with a tile that fits an nx x ny grid), for a
+        problem that runs alone."""
         if self._solo is None:
-            sw, st = solo_specs(self.plugin, self.precision, self.its, self.b)
-            msw, mst = kernels.load(sw), kernels.load(st)
-            f = {n: msw.get_function(n) for n in kernels.SWEPT_KERNELS}
-            self._solo = (sw, st, msw, mst, f, mst.get_function("Standard"))
-        return self._solo
+            sw = solo_specs(self.plugin, self.precision, self.its, self.b)[0]
+            msw = kernels.load(sw)
+            self._solo = (sw, msw, {n: msw.get_function(n) for n in kernels.SWEPT_KERNELS})
+        sw, msw, f = self._solo
+        st = solo_standard_spec(self.plugin, self.precision, self.its, nx, ny)
+        mst = kernels.load(st)
+        return sw, st, msw, mst, f, mst.get_function("Standard")
 
 
 class _Pool:
@@ -632,10 +653,10 @@ class Engine:
         ps_standard_run), without the batched kernels' per-work-item
         lookups."""
         cls = t.cls
-        sw, st, msw, mst, f, fstd = cls.solo_modules()
-        if cls.solo_consts is not t:
+        sw, st, msw, mst, f, fstd = cls.solo_modules(t.nx, t.ny)
+        if cls.solo_consts != (t, st):
             cls.cpu.set_globals(*t.problem.globals, source_mod=plugins.SourceModuleFacade([msw, mst]))
-            cls.solo_consts = t
+            cls.solo_consts = (t, st)
         G, P, T = t.times.G, cls.P, 2 * cls.P + cls.its
         c0, c1 = t.c, t.c + R
         if G >= 0 and c0 <= G + 1:

@@ -236,11 +236,14 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scr
     wins over rows of 32 values.  Measured on MI355X (Euler 1024^2, level A
     alone in LDS): 16x16 on 512 lanes 13.6 GCUPS, 8x32 on 512 lanes 13.4,
     8x16 on 256 lanes 12.8, 16x8 on 256 lanes 12.7 (with levels A-1 and A
-    in LDS, 256 lanes: 8x16 7.1, 8x32 6.6, 16x16 6.5)."""
+    in LDS, 256 lanes: 8x16 7.1, 8x32 6.6, 16x16 6.5).  Rows of 128 values
+    are tried with 512 lanes or more: heat 8192^2 32x128 on 512 lanes 318
+    GCUPS, 16x256 308, 32x64 on 256 lanes 300 (its one-value y halos cost
+    half a line per 64-value row), 32x128 on 256 lanes 225."""
     faces_shared = scratch is not None and scratch(0, 1) > scratch(0, 0)
     slots = its + 1 if slots is None else slots
     best = None
-    for by in (64, 32, 16, 8):
+    for by in ((128,) if threads >= 512 else ()) + (64, 32, 16, 8):
         for bx in (32, 16, 8, 4, 2):
             if bx > nx or by > ny:
                 continue
@@ -267,9 +270,9 @@ def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=Non
                   real="double", multi=False, mpss=1, generic=False):
     sc = None if generic or not level_update_in_standard(user_path) else level_scratch(user_path)
     if threads is None:
-        # a level update that shares faces through LDS gets 512 lanes, so
-        # that a 16x16 tile's passes fit one sweep (measured in standard_tile)
-        threads = 512 if sc is not None and sc(0, 1) > sc(0, 0) else 256
+        # a level update gets 512 lanes: a 16x16 Euler tile's passes fit one
+        # sweep, and heat takes 32x128 tiles (measured in standard_tile)
+        threads = 512 if sc is not None else 256
     if bx is None or by is None:
         bx, by = standard_tile(nv, ops, its, nx, ny, lds_budget, 8 if real == "double" else 4, sc, threads,
                                standard_slots(user_path, its, generic))
