@@ -16,7 +16,6 @@ the host takes the remaining block rows, swept only), the figure options
 the reference's quantization to whole swept cycles.
 """
 import argparse
-import math
 import sys
 import warnings
 
@@ -44,34 +43,24 @@ def adjust_array_size(args):
         warnings.warn(f"Adjusting array size to fit blocksize. New array size is {args.spacesteps}.")
 
 
-def euler_vortex(npx, npy, t=0.0, gamma=1.4, L=5.0):
+def euler_vortex(npx, npy, t=0.0, gamma=1.4):
     """Isentropic vortex sampled on the reference's CLI grid, endpoints
-    included (equations/euler.py:33-56 with one rank; analytical
-    euler.py:63-105: eps = 5, M = sqrt(2/gamma), free stream (1, 1))."""
-    x = numpy.linspace(-L, L, npx, endpoint=True)
-    y = numpy.linspace(-L, L, npy, endpoint=True)
-    X, Y = numpy.meshgrid(x, y, indexing="ij")
-    minf = math.sqrt(2 / gamma)
-    beta = minf * 5.0 * math.sqrt(2) / 4 / math.pi * math.exp(0.5)
-    vinf = minf * math.sqrt(gamma)
-    u0, v0 = vinf * math.cos(math.pi / 4), vinf * math.sin(math.pi / 4)
-    xc, yc = u0 * t, v0 * t
-    dx0, dy0 = X - xc, Y - yc
-    om = beta * numpy.exp(-0.5 * (dx0 * dx0 + dy0 * dy0))
-    dT = -(gamma - 1) * om * om / 2
-    rho = (1 + dT) ** (1 / (gamma - 1))
-    u, v = u0 - dy0 * om, v0 + dx0 * om
-    p = 1 / gamma * (1 + dT) ** (gamma / (gamma - 1))
-    return numpy.stack([rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)])
+    included (the Euler plugin's createInitialConditions grid,
+    equations/euler.py:33-56 with one rank)."""
+    from pysweep_amd.equations import euler
+    x = numpy.linspace(-5, 5, npx)
+    y = numpy.linspace(-5, 5, npy)
+    return numpy.stack(numpy.broadcast_arrays(*euler.analytical(x[:, None], y[None, :], t, gamma=gamma)))
 
 
 def heat_mode(npx, npy, t=0.0, alpha=1.0):
     """sin(2 pi x) sin(2 pi y) exp(-8 pi^2 alpha t) on the reference's CLI
-    grid (equations/heat.py:66-93 with one rank)."""
-    x = numpy.linspace(0, 1, npx, endpoint=True)
-    y = numpy.linspace(0, 1, npy, endpoint=True)
-    u = numpy.outer(numpy.sin(2 * numpy.pi * x), numpy.sin(2 * numpy.pi * y)) * math.exp(-8 * math.pi ** 2 * alpha * t)
-    return u[None]
+    grid (the heat plugin's createInitialConditions grid,
+    equations/heat.py:66-93 with one rank)."""
+    from pysweep_amd.equations import heat
+    x = numpy.linspace(0, 1, npx)
+    y = numpy.linspace(0, 1, npy)
+    return heat.analyticalEquation(x[:, None], y[None, :], t, alpha)[None]
 
 
 def _solver(args, ic, globals_, ops, its, cpu, gpu, output):

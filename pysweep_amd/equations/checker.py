@@ -31,3 +31,13 @@ def step(state, iidx, arrayTimeIndex, globalTimeStep):
     s = s + q[:, ii, (jj + 1) % ny]
     s = s + q[:, ii, jj - 1]
     state[t + 1][:, ii, jj] = s / 4
+
+
+def createInitialConditions(nv, nx, ny, filename="checkerConditions.hdf5"):
+    """A checkerboard of ones on the cells with i + j even, zeros elsewhere,
+    shape (nv, nx, ny), in dataset ``data`` of an HDF5 file
+    (checker.py:45-57); returns the file name."""
+    from pysweep_amd import hdf5io
+    i, j = numpy.indices((nx, ny))
+    board = ((i + j) % 2 == 0).astype(numpy.float64)
+    return hdf5io.write_datasets(filename, {"data": numpy.repeat(board[None], nv, axis=0)})

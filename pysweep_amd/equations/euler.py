@@ -100,3 +100,54 @@ def step(state, iidx, arrayTimeIndex, globalTimeStep):
             state[t + 1][:, ii, jj] = state[t - 1][:, ii, jj] + (fx * dtdx + fy * dtdy)
         else:
             state[t + 1][:, ii, jj] = state[t][:, ii, jj] + 0.5 * (fx * dtdx + fy * dtdy)
+
+
+# ------------------------------------------------------------------------
+# Initial conditions and the analytic solution (the reference plugin's
+# helpers, pysweep/equations/euler.py:33-116): the isentropic vortex of
+# strength 5 on [-5, 5]^2, free stream at Mach sqrt(2/gamma) along 45
+# degrees, advected without change of shape.
+
+def analytical(x, y, t, gamma=1.4, alpha=numpy.pi / 4):
+    """(rho, rho u, rho v, E) of the advected vortex at (x, y, t)
+    (euler.py:58-105); scalars or broadcastable arrays.  No periodic wrap
+    of the centre, as in the reference."""
+    mach = numpy.sqrt(2 / gamma)
+    beta = mach * 5 * numpy.sqrt(2) / 4 / numpy.pi * numpy.exp(0.5)
+    speed = mach * numpy.sqrt(gamma)  # free-stream speed, rho = p = 1
+    ddx = x - speed * numpy.cos(alpha) * t
+    ddy = y - speed * numpy.sin(alpha) * t
+    om = beta * numpy.exp(-0.5 * (ddx * ddx + ddy * ddy))
+    dT = -(gamma - 1) * om * om / 2
+    rho = (1 + dT) ** (1 / (gamma - 1))
+    u = speed * numpy.cos(alpha) - ddy * om
+    v = speed * numpy.sin(alpha) + ddx * om
+    p = (1 + dT) ** (gamma / (gamma - 1)) / gamma
+    return rho, rho * u, rho * v, p / (gamma - 1) + 0.5 * rho * (u * u + v * v)
+
+
+def createInitialConditions(npx, npy, t=0, gamma=1.4, filename="eulerConditions.hdf5"):
+    """Write the vortex at time t to an HDF5 file with dataset ``data`` of
+    shape (4, npx, npy) and return its name (euler.py:33-56).  The grid
+    includes both end points (linspace(-5, 5, n)), and the dataset is stored
+    as float32, as the reference's h5py default dataset is."""
+    from pysweep_amd import hdf5io
+    x = numpy.linspace(-5, 5, npx)
+    y = numpy.linspace(-5, 5, npy)
+    q = numpy.stack(numpy.broadcast_arrays(*analytical(x[:, None], y[None, :], t, gamma=gamma)))
+    return hdf5io.write_datasets(filename, {"data": q.astype(numpy.float32)})
+
+
+def getAnalyticalArray(npx, npy, t):
+    """The vortex at time t on the periodic grid -5 + 10 i/npx (endpoints
+    excluded), shape (4, npx, npy), with the module's gamma (euler.py:107-116)."""
+    x = numpy.linspace(-5, 5, npx, endpoint=False)
+    y = numpy.linspace(-5, 5, npy, endpoint=False)
+    return numpy.stack(numpy.broadcast_arrays(*analytical(x[:, None], y[None, :], t, gamma=gamma)))
+
+
+def getFinalTime(npx, timesteps, d=0.1, gamma=1.4):
+    """(final time, dt) for dt = d dx, dx = 10/npx (euler.py:58-61)."""
+    dx = 10 / npx
+    dt = d * dx
+    return timesteps * dt, dt

@@ -30,3 +30,10 @@ def step(state, iidx, arrayTimeIndex, globalTimeStep):
         state[t + 1][:, ii, jj] = state[t - 1][:, ii, jj] + 2
     else:
         state[t + 1][:, ii, jj] = state[t][:, ii, jj] + 1
+
+
+def createInitialConditions(nv, nx, ny, filename="exampleConditions.hdf5"):
+    """Zeros of shape (nv, nx, ny) in dataset ``data`` of an HDF5 file
+    (example.py:29-34); returns the file name."""
+    from pysweep_amd import hdf5io
+    return hdf5io.write_datasets(filename, {"data": numpy.zeros((nv, nx, ny))})

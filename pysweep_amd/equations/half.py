@@ -30,3 +30,12 @@ def step(state, iidx, arrayTimeIndex, globalTimeStep):
         state[t + 1, 0, ii, jj] = state[t - 1, 0, ii, jj] + 1 * d
     else:
         state[t + 1, 0, ii, jj] = state[t, 0, ii, jj] + 0.5 * d
+
+
+def createInitialConditions(npx, npy, t=0, filename="halfConditions.hdf5"):
+    """u = (flat index of the cell)/100 on an (npx, npy) grid, shape
+    (1, npx, npy), in dataset ``data`` of an HDF5 file (half.py:54-69);
+    returns the file name."""
+    from pysweep_amd import hdf5io
+    u = numpy.arange(npx * npy, dtype=numpy.float64).reshape(1, npx, npy) / 100
+    return hdf5io.write_datasets(filename, {"data": u})

@@ -47,3 +47,45 @@ def step(state, iidx, arrayTimeIndex, globalTimeStep):
         state[t + 1, 0, ii, jj] = 1 * d + state[t - 1, 0, ii, jj]
     else:
         state[t + 1, 0, ii, jj] = 0.5 * d + state[t, 0, ii, jj]
+
+
+# ------------------------------------------------------------------------
+# Initial conditions and the analytic solution (the reference plugin's
+# helpers, pysweep/equations/heat.py:66-120): the mode sin(2 pi x) sin(2 pi y)
+# of the periodic unit square decays as exp(-8 pi^2 alpha t).
+
+def analyticalEquation(x, y, t, alpha=0.1):
+    """u(x, y, t) of the decaying (2, 2) half-wave mode (heat.py:95-98);
+    scalars or broadcastable arrays."""
+    k = 2 * numpy.pi
+    return numpy.sin(k * x) * numpy.sin(k * y) * numpy.exp(-2 * k * k * alpha * t)
+
+
+def createInitialConditions(npx, npy, alpha=0.1, t=0, filename="heatConditions.hdf5"):
+    """Write the mode at time t to an HDF5 file with dataset ``data`` of
+    shape (1, npx, npy) and return its name (heat.py:66-93).  The grid
+    includes both end points (linspace(0, 1, n)), and the dataset is stored
+    as float32, as the reference's h5py default dataset is."""
+    from pysweep_amd import hdf5io
+    x = numpy.linspace(0, 1, npx)
+    y = numpy.linspace(0, 1, npy)
+    u = analyticalEquation(x[:, None], y[None, :], t, alpha)
+    return hdf5io.write_datasets(filename, {"data": u[None].astype(numpy.float32)})
+
+
+def analytical(npx, npy, t, alpha=0.1):
+    """(u, X, Y): the mode at time t on the periodic grid X = i/npx,
+    Y = j/npy (heat.py:100-113); u has shape (1, npx, npy)."""
+    X = numpy.linspace(0, 1, npx, endpoint=False)
+    Y = numpy.linspace(0, 1, npy, endpoint=False)
+    u = analyticalEquation(X[:, None], Y[None, :], t, alpha)
+    return u[None], X, Y
+
+
+def getFinalTime(npx, tsteps, alpha):
+    """(final time, dt) for tsteps steps of dt = 0.1 dx^2 / alpha with
+    dx = 1/npx.  As in the reference (heat.py:115-120) the alpha argument is
+    not used: the time step is computed with alpha = 0.01."""
+    dx = 1 / npx
+    dt = float(0.1 * dx ** 2 / 0.01)
+    return dt * tsteps, dt
