@@ -34,6 +34,9 @@ FLOPS_PER_CELL_STAGE = 626          # euler.cu:22-244 counted per stage (1252 pe
 FP64_PEAK_TFLOPS = 78.6             # MI355X FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_euler1024_b16.json")
+#: the kernel build the committed counter file was measured on (bumped when
+#: the swept or standard kernels change)
+KERNEL_BUILD = 1
 
 
 def parse():
@@ -48,7 +51,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["swept", "standard", "both"], default="both")
     ap.add_argument("--batch", type=int, default=512, help="problems of the batched workload (0: skip)")
-    ap.add_argument("--batch-legs", default="swept,standard,plain",
+    ap.add_argument("--batch-legs", default="tuned,swept,standard,plain",
                     help="runs of the batched workload: engine with swept / standard kernels, plain one at a time")
     ap.add_argument("--serve", type=int, default=48, help="requests of the served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=4.0, help="served-stream arrival rate per second")
@@ -125,18 +128,31 @@ def _cpu_strip(args):
     return q[:, halo:halo + (hi - lo)], time.perf_counter() - t0
 
 
+#: host cores a GPU may use on this machine (its share of the host's CPUs)
+HOST_CORES_PER_GPU = 16
+
+
+def host_cores():
+    """Cores this process may run on, at most the GPU's share of the host."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(HOST_CORES_PER_GPU, n))
+
+
 def cpu_baseline(ic, g, stages, workers=None):
     """numpy oracle (the reference's numpy step order) on the full grid for
-    a few stages, x-strips over `workers` processes (the host's cores, at
-    most 16); the result is checked against a one-process run of the first
-    stages on the same grid."""
+    a few stages, x-strips over `workers` processes (every core the GPU's
+    share of the host gives: host_cores()); the result is checked against a
+    one-process run of the first stages on the same grid."""
     import multiprocessing as mp
 
     import numpy as np
 
     from oracle import euler as o_euler
     dt, dx = g[2], g[3]
-    workers = workers or max(1, min(16, os.cpu_count() or 1))
+    workers = workers or host_cores()
     nv, nx, ny = ic.shape
     halo = 2 * stages
     bounds = [(nx * i // workers, nx * (i + 1) // workers) for i in range(workers)]
@@ -166,45 +182,87 @@ def cpu_baseline(ic, g, stages, workers=None):
             "np_version": np.__version__}
 
 
-def batched_workload(n, seed=0, legs=("swept", "standard", "plain")):
+def _size_bucket(p):
+    cells = p.ic.shape[1] * p.ic.shape[2]
+    return "small(<=256^2)" if cells <= 256 * 256 else ("medium(<=640^2)" if cells <= 640 * 640 else "large")
+
+
+def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "plain"), buckets=True):
     """Independent heat and Euler problems, 200 steps each, FP64: the batched
     engine (instance-table launches, graph-replayed rounds) against solving
-    them one at a time with the plain standard kernel.  ``legs`` narrows the
-    study to some of the three runs (a profile of one engine run uses it)."""
-    from pysweep_amd import baseline, batch, hip, workloads
+    them one at a time with the plain standard kernel.
+
+    The "tuned" engine is the default one after ``Tuner.tune_classes`` has
+    picked each class's block size, method and workgroup shapes on a sample
+    of its problems (40 steps each; the time it took is reported); the
+    "swept" and "standard" engines force one method with the default
+    shapes.  ``buckets`` adds a per-size breakdown: each size bucket run
+    alone on the tuned engine and one at a time.  ``legs`` narrows the study
+    (a profile of one engine run uses it)."""
+    from pysweep_amd import autotune, baseline, batch, hip, workloads
     probs = workloads.batched_mix(n=n, steps=200, seed=seed)
     cells = sum(p.ic.shape[1] * p.ic.shape[2] * 200 for p in probs)
     out = {"problems": n, "steps": 200, "cell_steps": cells}
     results = {}
-    for label, method in (("engine_swept", "swept"), ("engine_standard", "standard")):
-        if method not in legs:
-            continue
-        for p in probs:
+    tuner = None
+    if "tuned" in legs:
+        tuner = autotune.Tuner(path=None)
+        t0 = time.perf_counter()
+        recs = tuner.tune_classes(probs, sample=24, steps=40, save=False)
+        out["tuning"] = {"seconds": time.perf_counter() - t0, "sample_problems_per_class": 24, "sample_steps": 40,
+                         "choices": {k: {kk: v for kk, v in r.items() if kk != "trials"} for k, r in recs.items()},
+                         "trials": {k: [{kk: v for kk, v in t.items() if kk in ("b", "method", "slots", "stile",
+                                                                                "sthreads", "gcups",
+                                                                                "workgroups_per_cu")}
+                                        for t in r["trials"]] for k, r in recs.items()}}
+
+    def engine_run(ps, method, tuner_):
+        for p in ps:
             p.method = method
-        eng = batch.Engine()
-        for p in probs:
+        eng = batch.Engine(tuner=tuner_)
+        for p in ps:
             eng.submit(p, arrival=0.0)
         hip.synchronize()
         t0 = time.perf_counter()
         eng.run()
         hip.synchronize()
-        sec = time.perf_counter() - t0
+        return time.perf_counter() - t0, eng
+
+    for label, method, tu in (("engine_tuned", "auto", tuner), ("engine_swept", "swept", None),
+                              ("engine_standard", "standard", None)):
+        if label.split("_")[1] not in legs:
+            continue
+        sec, eng = engine_run(probs, method, tu)
         results[label] = {t.problem.tag: t.result for t in eng.finished}
+        drift = max(t.conservation["drift"] for t in eng.finished)
         out[label] = {"seconds": sec, "solves_per_s": n / sec, "gcups": cells / sec / 1e9, "rounds": eng.rounds,
-                      "peak_hbm_gb": eng.peak_in_use / 1e9}
+                      "peak_hbm_gb": eng.peak_in_use / 1e9, "max_conserved_sum_drift": drift}
     if "plain" not in legs:
         return out
+    per = {}
     t0 = time.perf_counter()
     same = 0
     for p in probs:
+        t1 = time.perf_counter()
         r = baseline.solve_plain(p)
+        per[p.tag] = time.perf_counter() - t1
         same += int(all(numpy_equal(r, res[p.tag]) for res in results.values()))
     sec = time.perf_counter() - t0
     out["one_at_a_time_plain"] = {"seconds": sec, "solves_per_s": n / sec, "gcups": cells / sec / 1e9}
     out["bitwise_equal_results"] = same
     if results:
-        best = max(out[k]["solves_per_s"] for k in results)
-        out["engine_over_plain"] = best / out["one_at_a_time_plain"]["solves_per_s"]
+        key = "engine_tuned" if "engine_tuned" in out else max(results, key=lambda k: out[k]["solves_per_s"])
+        out["engine_over_plain"] = out[key]["solves_per_s"] / out["one_at_a_time_plain"]["solves_per_s"]
+        out["engine_over_plain_leg"] = key
+    if buckets and tuner is not None:
+        rows = {}
+        for name in sorted({_size_bucket(p) for p in probs}):
+            ps = [p for p in probs if _size_bucket(p) == name]
+            esec, _ = engine_run(ps, "auto", tuner)
+            psec = sum(per[p.tag] for p in ps)
+            rows[name] = {"problems": len(ps), "cell_steps": sum(p.ic.shape[1] * p.ic.shape[2] * 200 for p in ps),
+                          "engine_s": esec, "plain_s": psec, "engine_over_plain": psec / esec}
+        out["size_buckets"] = rows
     return out
 
 
@@ -341,6 +399,35 @@ def precision_sensitivity(n=1024, b=16, steps=(10, 100, 1000)):
             rec[k + "_rel_Linf"] = float(np.abs(e).max() / np.abs(ref).max())
         out.append(rec)
     return out
+
+
+def product_host_path(n=1024, b=16, steps=8):
+    """The product's own host path on a prefix of the headline workload: the
+    Euler vortex at n^2 with share = 0, i.e. the swept decomposition computed
+    on the host, level by level with the plugin's Python step over each
+    phase's cells (the reference's CPU path, core/process.py:81-111 and
+    core/geometry.py:26-36), one process.  ``steps`` steps with the
+    reference's step quantization (b=16: P=3; 8 steps give G=4 and 15 stages)."""
+    from pysweep_amd import Solver, workloads
+    from pysweep_amd.equations import path as eq_path
+    dx = 10.0 / n
+    dt = 0.1 * dx
+    ic = workloads.vortex_state(n, n)
+    sol = Solver(ic, sendWarning=False)
+    sol.simulation, sol.operating, sol.intermediate, sol.blocksize = True, 2, 2, (b, b, 1)
+    sol.share = 0.0
+    sol.exact = False
+    sol.globals = [0.0, steps * dt, dt, dx, dx, 1.4]
+    sol.setCPU(eq_path("euler.py"))
+    sol.setGPU(eq_path("euler.cu"))
+    sol.loadCPUModule()
+    sol()
+    cell_steps = n * n * sol.stages / 2
+    return {"value": cell_steps / sol.solve_seconds / 1e9, "unit": "GCUPS", "cores": 1, "kind": "port",
+            "sample": f"share=0 swept solve of the {n}x{n} Euler vortex on the host (the product's CPU path: "
+                      f"the plugin's Python step over each phase level's cells), b={b}, {sol.stages} RK2 stages "
+                      f"({sol.stages // 2} steps), one process, {sol.solve_seconds:.2f} s",
+            "seconds": sol.solve_seconds}
 
 
 def heat_cpu_case():
@@ -500,8 +587,14 @@ def main():
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
-            traffic = json.load(f).get(kern, {}).get("bytes_per_launch")
-    out["roofline"] = {"bound": "mfma", "pipe": "fp64-valu", "kernel": kern, "achieved": achieved,
+            tf = json.load(f)
+        # counted bytes apply only to the workload and kernels they were counted on
+        wl = tf.get("_workload", {})
+        if (wl.get("n"), wl.get("b"), wl.get("build")) == (a.n, a.b, KERNEL_BUILD) and not a.generic:
+            traffic = tf.get(kern, {}).get("bytes_per_launch")
+    # the Euler step is FP64 vector arithmetic (a stencil, no contraction):
+    # the roof is the FP64 vector pipe, 78.6 TFLOP/s on MI355X
+    out["roofline"] = {"bound": "fp64-valu", "kernel": kern, "achieved": achieved,
                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                        "traffic": traffic, "avg_launch_ms": stats[kern]["avg_ms"],
                        "cell_stages_per_launch": cs, "flops_per_cell_stage": FLOPS_PER_CELL_STAGE,
@@ -531,6 +624,7 @@ def main():
         out["p99_latency_s"] = out["serving"]["engine"]["p99_s"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, g, a.cpu_stages)
+        out["cpu_baseline"]["product_host_path"] = product_host_path(a.n, a.b)
     print(json.dumps(out), flush=True)
 
 

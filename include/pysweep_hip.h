@@ -13,8 +13,12 @@
  *   - host buffers are caller-owned and only borrowed for the call;
  *   - device buffers and modules are owned by the caller through the
  *     handles returned here and released with ps_free / ps_module_unload;
- *   - all device work is ordered on one HIP stream per process (created by
- *     ps_init, default priority); calls are not thread-safe.
+ *   - device work is ordered on the work stream of the calling thread's
+ *     current context: the one made current by ps_context_push, else the
+ *     process context of ps_init.  Each context has its own stream (default
+ *     priority) and launch timer.  One context must not be used by two
+ *     threads at once; distinct contexts may be driven from distinct threads
+ *     (a server solving several problems at once).
  */
 #ifndef PYSWEEP_HIP_H
 #define PYSWEEP_HIP_H
@@ -29,6 +33,7 @@ typedef void *ps_module;   /* loaded code object (hipModule_t)   */
 typedef void *ps_function; /* kernel handle (hipFunction_t)       */
 typedef void *ps_event;    /* timing event (hipEvent_t)           */
 typedef void *ps_graph;    /* instantiated graph (hipGraphExec_t) */
+typedef void *ps_context;  /* device + work stream + launch timer */
 
 /* ABI version of this header. */
 int ps_version(void);
@@ -46,6 +51,23 @@ int ps_init(int device);
 /* Release the stream.  Replaces cuda_context.pop() (core/process.py:31-32). */
 int ps_shutdown(void);
 
+/* Contexts of their own: a device and a work stream per handle, made current
+ * per thread.  Replace cuda.Device(rank).make_context() / context.push() /
+ * context.pop() (core/block.py:36-38; core/process.py:31-32), which the
+ * reference holds one per MPI rank.  ps_context_create does not make the
+ * context current; ps_context_push does (on the calling thread, over any
+ * context already current there) and ps_context_pop restores the previous
+ * one.  Modules loaded while a context is current belong to its device.
+ * ps_context_destroy drains the stream first and fails for a context current
+ * on the calling thread. */
+int ps_context_create(int device, ps_context *ctx);
+int ps_context_destroy(ps_context ctx);
+int ps_context_push(ps_context ctx);
+int ps_context_pop(void);
+/* The context current on the calling thread, NULL when it uses the process
+ * context. */
+int ps_context_current(ps_context *ctx);
+
 /* Compile HIP source text for gfx950 with hiprtc; no device is needed.
  * On success *code (size *size) is a code object the caller frees with
  * ps_free_host; *log (may be NULL on success) is the compiler log, also
@@ -54,6 +76,10 @@ int ps_shutdown(void);
 int ps_compile(const char *source, const char *name, const char *const *options, int noptions,
                char **code, size_t *size, char **log);
 void ps_free_host(void *p);
+
+/* Version of the hiprtc library that ps_compile uses (part of the key under
+ * which compiled code objects are cached; no device is needed). */
+int ps_compiler_version(int *major, int *minor);
 
 /* Load / unload a code object on the current device. */
 int ps_module_load(const void *code, size_t size, ps_module *module);
@@ -102,7 +128,7 @@ int ps_memcpy_dtod_async(void *dst, const void *src, size_t bytes);
 int ps_launch(ps_function function, unsigned gx, unsigned gy, unsigned gz, unsigned bx, unsigned by,
               unsigned bz, unsigned shared_bytes, void **args);
 
-/* Wait for the work stream.  Replaces cuda.Context.synchronize()
+/* Wait for the current context's work stream.  Replaces cuda.Context.synchronize()
  * (core/functions.py:31, 60, 84, 142). */
 int ps_synchronize(void);
 
@@ -246,6 +272,16 @@ int ps_multi_run_timed(const ps_multi_plan *plan, int rounds, ps_run_stats *stat
  * stream (ring resets, and the reference copy of known size that calibrates
  * HBM byte counters). */
 int ps_copy_f64(void *dst, const void *src, size_t n);
+
+/* FP64 sums of each variable of one level level[nv][cells] (FP64 storage,
+ * or FP32 when fp32 != 0), stream ordered: sums[v] = sum of variable v and
+ * sums[nv + v] = sum of its magnitudes, for v < nv <= 8, into device memory
+ * sums[2 * nv].  The conserved integrals (mass, momentum, energy; the heat
+ * content) of the FP32-state mode are summed in FP64 here.  The order of
+ * the additions is fixed, so a level gives the same bits on every call.
+ * Partial sums go to a buffer the context allocates on first use.  No
+ * counterpart in the reference, which checks no conserved quantity. */
+int ps_level_sums(const void *level, int fp32, int nv, size_t cells, double *sums);
 
 /* ---------------------------------------------------------------------- */
 /* Graph capture of the work stream (no counterpart in the reference).     */

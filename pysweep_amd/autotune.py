@@ -1,11 +1,15 @@
 """Autotuner for instance-table classes: method (swept or standard), block
-size, LDS window, workgroup size and persistent blocks per CU.
+size b, and the occupancy of each kernel family (workgroups per CU, set
+through the swept tile's LDS window and lanes, and the standard tile's
+shape and lanes).
 
-``Tuner.tune`` runs a small sample batch with each candidate launch shape
-and keeps the fastest (cell-steps per second); every candidate computes the
-same per-cell arithmetic, so the choice never changes results (checked by
-the GPU tests).  Choices persist in a JSON file next to the code-object
-cache; a class without a record uses the measured defaults below.
+``Tuner.tune`` runs a small sample batch with each candidate and keeps the
+fastest (cell-steps per second); every candidate computes the same per-cell
+arithmetic, so the choice never changes results (checked by the GPU tests).
+Each trial records the kernels' registers and LDS bytes and the workgroups
+per CU they allow.  Choices persist in a JSON file next to the code-object
+cache; ``batch.Engine`` reads them by default, and a class without a record
+uses the measured defaults below.
 """
 import json
 import os
@@ -27,10 +31,47 @@ DEFAULTS = {
     ("heat", "fp32"): {"b": 32},
 }
 
+#: the launch-shape keys of a candidate besides b and method
+SHAPE_KEYS = ("slots", "threads", "stile", "sthreads")
+
+#: candidates per plugin: the swept block sizes, the swept LDS window (a
+#: smaller window fits more workgroups on a CU), and the standard kernel with
+#: tiles of 256 or 512 lanes (8 or 4 workgroups of LDS per CU for Euler)
 CANDIDATES = {
-    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "method": "standard"}],
-    "heat": [{"b": 16}, {"b": 32}, {"b": 64}, {"b": 32, "method": "standard"}],
+    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "slots": 3},
+              {"b": 16, "method": "standard"},
+              {"b": 16, "method": "standard", "stile": [8, 16], "sthreads": 256},
+              {"b": 16, "method": "standard", "stile": [8, 32], "sthreads": 512}],
+    "heat": [{"b": 16}, {"b": 32}, {"b": 64}, {"b": 32, "slots": 2},
+             {"b": 32, "method": "standard"},
+             {"b": 32, "method": "standard", "stile": [32, 64], "sthreads": 256},
+             {"b": 32, "method": "standard", "stile": [16, 128], "sthreads": 512}],
 }
+
+#: per CU on MI355X: LDS bytes, SIMDs, wavefront slots per SIMD, VGPRs per lane
+CU_LDS = 160 * 1024
+CU_SIMDS = 4
+SIMD_WAVES = 8
+SIMD_VGPRS = 512
+
+
+def shape_of(cand):
+    """The launch-shape keyword arguments of a candidate or record."""
+    out = {k: cand[k] for k in SHAPE_KEYS if k in cand}
+    if "stile" in out:
+        out["stile"] = tuple(out["stile"])
+    return out
+
+
+def workgroups_per_cu(lds_bytes, regs, threads):
+    """Resident workgroups per CU of a kernel: by LDS, and by registers
+    (waves of 64 lanes share the CU's 4 SIMDs; each SIMD holds up to 8 waves
+    and 512 VGPRs per lane, allocated in blocks of 8)."""
+    waves = max(1, -(-threads // 64))
+    by_lds = CU_LDS // max(lds_bytes, 1)
+    per_simd = SIMD_WAVES if regs <= 0 else min(SIMD_WAVES, SIMD_VGPRS // (-(-regs // 8) * 8))
+    by_regs = (per_simd * CU_SIMDS) // waves
+    return int(min(by_lds, by_regs))
 
 
 def _key(plugin, precision, its):
@@ -65,13 +106,10 @@ class Tuner:
         return r.get("method", batch.DEFAULT_METHOD.get((plugin, precision), "swept"))
 
     def config(self, plugin, precision, its, b):
+        """Launch shapes of the class (plugin, precision, its, b): the
+        record's, when it was tuned with this b."""
         r = self.record(plugin, precision, its)
-        out = {}
-        if r.get("b") == b:
-            for k in ("slots", "threads"):
-                if k in r:
-                    out[k] = r[k]
-        return out
+        return shape_of(r) if r.get("b") == b else {}
 
     def tune(self, plugin, precision, its, make_problems, candidates=None, save=True):
         """Time each candidate on ``make_problems()`` (a list of Problems);
@@ -89,7 +127,7 @@ class Tuner:
                     return cand["b"]
 
                 def config(self_, *a):
-                    return {k: v for k, v in cand.items() if k not in ("b", "method")}
+                    return shape_of(cand)
 
             eng = batch.Engine(tuner=_Fixed(), max_rounds_ahead=1)
             for p in probs:
@@ -105,11 +143,14 @@ class Tuner:
             for t in eng.finished:
                 nv, nx, ny = t.problem.ic.shape
                 cells += nx * ny * (t.times.stages + t.times.tail) / t.problem.its
-            trials.append({"method": "swept", **cand, "seconds": sec, "gcups": cells / sec / 1e9,
+            method = cand.get("method", "swept")
+            occ = self._occupancy(eng, plugin, precision, its, cand["b"], method)
+            trials.append({"method": method, **cand, "seconds": sec, "gcups": cells / sec / 1e9,
+                           "workgroups_per_cu": occ,
                            "results": {t.problem.tag: t.result for t in eng.finished}})
         best = max(trials, key=lambda r: r["gcups"])
         self.last_trials = trials
-        rec = {k: v for k, v in best.items() if k in ("b", "slots", "threads", "method")}
+        rec = {k: v for k, v in best.items() if k in ("b", "method", "workgroups_per_cu") + SHAPE_KEYS}
         trials = [{k: v for k, v in t.items() if k != "results"} for t in trials]
         rec["trials"] = trials
         self.records[_key(plugin, precision, its)] = rec
@@ -118,3 +159,44 @@ class Tuner:
             with open(self.path, "w") as f:
                 json.dump(self.records, f, indent=1)
         return rec
+
+    @staticmethod
+    def _occupancy(eng, plugin, precision, its, b, method):
+        """Workgroups per CU of the class's kernel that carries the work:
+        the Octahedron (swept) or the standard kernel."""
+        from pysweep_amd import hip
+        cls = eng.classes.get((plugin, precision, its, b))
+        if cls is None:
+            return None
+        fn, threads = ((cls.fn["Octahedron"], cls.spec.threads) if method == "swept"
+                       else (cls.fn["Standard"], cls.sspec.threads))
+        info = hip.function_info(fn)
+        return workgroups_per_cu(info["lds_bytes"], info["regs"], threads)
+
+    def tune_classes(self, problems, sample=24, steps=40, seed=0, save=True):
+        """Tune every (plugin, precision, its) class that occurs in
+        ``problems`` on a sample of its own problems (up to ``sample`` of
+        them, shortened to ``steps`` steps); returns {class key: record}."""
+        import dataclasses
+        import random
+        groups = {}
+        for p in problems:
+            groups.setdefault((p.plugin, p.precision, p.its), []).append(p)
+        out = {}
+        rng = random.Random(seed)
+        for (plugin, precision, its), ps in sorted(groups.items()):
+            pick = rng.sample(ps, min(sample, len(ps)))
+
+            def make(pick=pick):
+                probs = []
+                for i, p in enumerate(pick):
+                    t0, dt = p.globals[0], p.globals[2]
+                    g = list(p.globals)
+                    g[1] = t0 + steps * dt
+                    probs.append(dataclasses.replace(p, globals=g, tag=i, method="swept", b=None))
+                return probs
+
+            cands = [c for c in CANDIDATES[plugin]
+                     if all(p.ic.shape[1] % c["b"] == 0 and p.ic.shape[2] % c["b"] == 0 for p in pick)]
+            out[_key(plugin, precision, its)] = self.tune(plugin, precision, its, make, candidates=cands, save=save)
+        return out

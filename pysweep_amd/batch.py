@@ -85,15 +85,19 @@ class _HostConsts:
         raise KeyError(name)
 
 
-def class_specs(plugin, precision, its, b, slots=None, threads=None):
-    """(swept, standard) instance-table module specs of a class."""
+def class_specs(plugin, precision, its, b, slots=None, threads=None, stile=None, sthreads=None):
+    """(swept, standard) instance-table module specs of a class: ``slots``
+    and ``threads`` shape the swept tile's workgroup (LDS window levels,
+    lanes), ``stile`` = (rows, columns) and ``sthreads`` the standard one;
+    None takes the measured defaults (the standard tile of a 4b x 4b grid)."""
     py, cu64, cu32, nv, ops, _ = PLUGINS[plugin]
     real = "double" if precision == "fp64" else "float"
-    rb = 8 if precision == "fp64" else 4
     cu = eq_path(cu64 if precision == "fp64" else cu32)
     P = b // (2 * ops) - 1
     spec = kernels.swept_spec(cu, nv, ops, its, b, slots=slots, threads=threads, real=real, multi=True)
-    sspec = kernels.standard_spec(cu, nv, ops, its, b * 4, b * 4, real=real, multi=True, mpss=P)
+    bx, by = stile if stile else (None, None)
+    sspec = kernels.standard_spec(cu, nv, ops, its, max(b * 4, bx or 0), max(b * 4, by or 0), bx=bx, by=by,
+                                  threads=sthreads, real=real, multi=True, mpss=P)
     return spec, sspec
 
 
@@ -127,6 +131,7 @@ def solo_specs(plugin, precision, its, b):
 def shipped_class_specs():
     """Instance-table modules (and their one-problem modules) used by the
     tests and benchmarks."""
+    from pysweep_amd import autotune
     out = []
     for plugin, precision, its, b in (("euler", "fp64", 2, 8), ("euler", "fp64", 2, 16), ("euler", "fp64", 2, 32),
                                       ("euler", "fp32", 2, 16), ("euler", "fp32", 2, 32),
@@ -134,20 +139,29 @@ def shipped_class_specs():
                                       ("heat", "fp64", 1, 64), ("heat", "fp32", 1, 16), ("heat", "fp32", 1, 32)):
         out.extend(class_specs(plugin, precision, its, b))
         out.extend(solo_specs(plugin, precision, its, b))
-    return out
+    # the autotuner's candidates of the batched workload's classes
+    for plugin, precision, its in (("euler", "fp64", 2), ("heat", "fp64", 1), ("heat", "fp64", 2)):
+        for cand in autotune.CANDIDATES[plugin]:
+            out.extend(class_specs(plugin, precision, its, cand["b"], **autotune.shape_of(cand)))
+    seen, uniq = set(), []
+    for sp in out:
+        if sp not in seen:
+            seen.add(sp)
+            uniq.append(sp)
+    return uniq
 
 
 class PluginClass:
     """Compiled instance-table modules of one (plugin, precision, b)."""
 
-    def __init__(self, plugin, precision, its, b, slots=None, threads=None):
+    def __init__(self, plugin, precision, its, b, slots=None, threads=None, stile=None, sthreads=None):
         py, cu64, cu32, nv, ops, default_its = PLUGINS[plugin]
         self.plugin, self.precision, self.nv, self.ops, self.its, self.b = plugin, precision, nv, ops, its, b
         self.real = "double" if precision == "fp64" else "float"
         self.dtype = numpy.float64 if precision == "fp64" else numpy.float32
         self.rb = numpy.dtype(self.dtype).itemsize
         self.P = b // (2 * ops) - 1
-        self.spec, self.sspec = class_specs(plugin, precision, its, b, slots, threads)
+        self.spec, self.sspec = class_specs(plugin, precision, its, b, slots, threads, stile, sthreads)
         self.layout = self.spec.const_layout()
         m = kernels.load(self.spec)
         sm = kernels.load(self.sspec)
@@ -157,10 +171,14 @@ class PluginClass:
         self.cpu = plugins.load_cpu_module(eq_path(py), name=f"pysweep_batch_{plugin}")
         self.key = (plugin, precision, its, b)
         self._solo = None
-        self.solo_consts = None  # problem whose constants the solo modules hold
 
-    def ring_bytes(self, nx, ny):
-        return (2 * self.P + self.its) * self.nv * nx * ny * self.rb
+    def levels(self, method):
+        """Ring levels of a problem: the swept window 2P+s (core/solver.py:123),
+        or s+1 for a problem that runs every stage on the standard kernel."""
+        return 2 * self.P + self.its if method == "swept" else self.its + 1
+
+    def ring_bytes(self, nx, ny, method="swept"):
+        return self.levels(method) * self.nv * nx * ny * self.rb
 
     def solo_modules(self, nx, ny):
         """The class's one-problem modules (the plain swept phases and the
@@ -278,9 +296,16 @@ class _StagingRing:
 
 
 def _fetch(item):
-    """A finished ticket's result out of its staging piece (FP64 copy)."""
+    """A finished ticket's result out of its staging piece (FP64 copy), and
+    its conserved-sum record from the device sums of its first and final
+    levels."""
+    from pysweep_amd import device
     t, (piece, dtype, shape) = item
     t.result = piece.view(dtype, shape).astype(numpy.float64)
+    nv = shape[0]
+    lvl = t.result.size * numpy.dtype(dtype).itemsize
+    s = _Slice(piece.buf, piece.offset + lvl, 256).view(numpy.float64, (2, 16))
+    t.conservation = device.conservation((s[0, :nv], s[0, nv:2 * nv]), (s[1, :nv], s[1, nv:2 * nv]))
 
 
 _COPY_POOL = []
@@ -307,6 +332,11 @@ def _take_ring(nbytes=0):
     return _StagingRing(nbytes)
 
 
+#: bytes after a problem's ring: its constant block (256) and the FP64 sums
+#: of its first and final levels (128 each: 2 * nv <= 16 doubles)
+_TAIL_BYTES = 512
+
+
 @dataclasses.dataclass(eq=False)
 class Ticket:
     problem: Problem
@@ -314,8 +344,10 @@ class Ticket:
     nx: int = 0
     ny: int = 0
     times: object = None
-    ring: object = None  # device buffer: the ring, then the constant block
+    ring: object = None  # device buffer: the ring, the constant block, the level sums
     consts: int = 0      # device address of the constant block
+    sums: int = 0        # device address of the FP64 level sums: first level, then final
+    conservation: dict = None  # conserved-sum record of the result (device.conservation)
     c: int = 0
     c_end: int = 0
     final_level: int = 0
@@ -325,6 +357,7 @@ class Ticket:
     finished: float = None
     nbytes: int = 0
     method: str = None
+    T: int = 0           # ring levels
     cells: int = 0
     last_active: float = 0.0
     boosted: bool = False
@@ -406,7 +439,7 @@ class _ClassTable:
         for i, t in enumerate(self.running):
             st = arr[i]
             st.ring, st.consts = t.ring.ptr, t.consts
-            st.nx, st.ny, st.T = t.nx, t.ny, 2 * cls.P + cls.its
+            st.nx, st.ny, st.T = t.nx, t.ny, t.T
             st.tiles_x, st.tiles_y = t.nx // cls.b, t.ny // cls.b
             st.stiles_x, st.stiles_y = -(-t.nx // sbx), -(-t.ny // sby)
             st.G, st.tail, st.c = t.times.G, t.times.tail, t.c
@@ -502,9 +535,15 @@ class Engine:
     graphs replayed from Python were 1-2 % slower than the C loop on the
     saturated served stream; graphs of 8 or 32 rounds run level with it
     (17.05-17.12 solves/s against 16.97-17.10; 512-problem mix 193-195
-    against 191-195)."""
+    against 191-195).
 
-    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, tuner=None, clock=None,
+    ``tuner``: the ``autotune.Tuner`` whose records give each class its
+    block size, the method of problems submitted with method="auto" and the
+    workgroup shapes of its kernels; "default" loads the stored records
+    (falling back to the measured defaults), None takes the first block
+    size that divides the grid and the default shapes."""
+
+    def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, tuner="default", clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
                  capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4):
         hip.ensure_init(0)
@@ -526,6 +565,9 @@ class Engine:
         self.graph_rounds = graph_rounds
         self.poll_rounds = poll_rounds
         self.max_rounds_ahead = max_rounds_ahead
+        if tuner == "default":
+            from pysweep_amd import autotune
+            tuner = autotune.Tuner()  # the stored records, else the measured defaults
         self.tuner = tuner
         self.classes = {}
         self.tables = {}
@@ -654,10 +696,13 @@ class Engine:
         lookups."""
         cls = t.cls
         sw, st, msw, mst, f, fstd = cls.solo_modules(t.nx, t.ny)
-        if cls.solo_consts != (t, st):
+        # Modules are shared by every class, engine and Solver that load the
+        # same spec (kernels.load), so the owner of their constants is kept
+        # on the modules themselves; any constant write clears it.
+        if msw.const_owner is not t or mst.const_owner is not t:
             cls.cpu.set_globals(*t.problem.globals, source_mod=plugins.SourceModuleFacade([msw, mst]))
-            cls.solo_consts = (t, st)
-        G, P, T = t.times.G, cls.P, 2 * cls.P + cls.its
+            msw.const_owner = mst.const_owner = t
+        G, P, T = t.times.G, cls.P, t.T
         c0, c1 = t.c, t.c + R
         if G >= 0 and c0 <= G + 1:
             plan = hip.SweptPlan(state=t.ring.ptr, nx=t.nx, ny=t.ny, b=cls.b, ops=cls.ops, its=cls.its, T=T,
@@ -758,7 +803,7 @@ class Engine:
         cls = self.classes.get(key)
         if cls is None:
             cfg = self.tuner.config(p.plugin, p.precision, p.its, b) if self.tuner else {}
-            cls = PluginClass(p.plugin, p.precision, p.its, b, slots=cfg.get("slots"), threads=cfg.get("threads"))
+            cls = PluginClass(p.plugin, p.precision, p.its, b, **cfg)
             self.classes[key] = cls
             self.tables[key] = _ClassTable(cls)
         return cls
@@ -793,32 +838,37 @@ class Engine:
             b = self._block_size(p)
             cls = self._class_for(p, b)
             nv, nx, ny = p.ic.shape
-            need = cls.ring_bytes(nx, ny) + 256
+            method = self._method_for(p)
+            need = cls.ring_bytes(nx, ny, method) + _TAIL_BYTES
             if self.in_use + need > self.hbm_budget:
                 if not self.running and need > self.hbm_budget:
                     raise MemoryError(f"problem needs {need} B, more than the HBM budget {self.hbm_budget} B")
                 break
             self.queue.remove(t)
-            self._start(t, cls, b, need)
+            self._start(t, cls, b, need, method)
             staged += p.ic.size * cls.rb
 
-    def _start(self, t, cls, b, need):
+    def _method_for(self, p):
+        """The kernel family a problem runs on ("auto": the tuner's record,
+        else the default table)."""
+        if p.method != "auto":
+            return p.method
+        return (self.tuner.method(p.plugin, p.precision, p.its) if self.tuner
+                else DEFAULT_METHOD.get((p.plugin, p.precision), "swept"))
+
+    def _start(self, t, cls, b, need, method):
         p = t.problem
         nv, nx, ny = p.ic.shape
         if nv != cls.nv or nx % b or ny % b:
             raise ValueError(f"problem shape {p.ic.shape} does not fit class {cls.key}")
         N = schedule.requested_steps(p.globals[0], p.globals[1], p.globals[2])
-        method = p.method
-        if method == "auto":
-            method = (self.tuner.method(p.plugin, p.precision, p.its) if self.tuner
-                      else DEFAULT_METHOD.get((p.plugin, p.precision), "swept"))
         t.method = method
         if method == "standard":
             t.times = schedule.standard_times(N, b, cls.ops, cls.its)
         else:
             t.times = schedule.swept_times(N, b, cls.ops, cls.its, exact=p.exact)
         t.nx, t.ny = nx, ny
-        T = 2 * cls.P + cls.its
+        T = t.T = cls.levels(method)
         lvl = nv * nx * ny * cls.rb
         hc = _HostConsts(cls.layout)
         cls.cpu.set_globals(*p.globals, source_mod=hc)
@@ -827,6 +877,7 @@ class Engine:
         # of 256 bytes: nx and ny are multiples of b >= 8)
         t.ring = self.rings.take(need, limit=self.hbm_budget - self.in_use)
         t.consts = t.ring.ptr + T * lvl
+        t.sums = t.consts + 256
         # levels 0 .. its-1 hold the initial condition: one upload from a
         # staging buffer, then device copies
         host = self._stage(lvl + 256)
@@ -836,6 +887,7 @@ class Engine:
         host.upload_async(t.consts, len(hc.buf), lvl)
         for A in range(1, cls.its):
             hip.dtod_async(t.ring.ptr + (A % T) * lvl, t.ring.ptr, lvl)
+        hip.level_sums_async(t.ring.ptr, cls.rb == 4, nv, nx * ny, t.sums)
         G = t.times.G
         t.c = 0 if G >= 0 else 1
         t.c_end = G + 2 + t.times.tail
@@ -859,14 +911,17 @@ class Engine:
         work on the stream that reuses it runs after the copy."""
         cls = t.cls
         nv = cls.nv
-        T = 2 * cls.P + cls.its
+        T = t.T
         lvl = nv * t.nx * t.ny * cls.rb
         self._fence()  # what is staged so far completes on its own
-        piece = self._stage(lvl)
-        piece.download_async(t.ring.ptr + (t.final_level % T) * lvl, lvl)
+        fin = t.ring.ptr + (t.final_level % T) * lvl
+        hip.level_sums_async(fin, cls.rb == 4, nv, t.nx * t.ny, t.sums + 128)
+        piece = self._stage(lvl + 256)  # the level, then its sums (lvl is a multiple of 256)
+        piece.download_async(fin, lvl)
+        piece.download_async(t.sums, 256, lvl)
         self._fence(t, (piece, cls.dtype, (nv, t.nx, t.ny)))
         self.rings.give(t.ring)
-        t.ring, t.consts = None, 0
+        t.ring, t.consts, t.sums = None, 0, 0
         self.in_use -= t.nbytes
         self.running.remove(t)
         tab = self.tables[cls.key]

@@ -17,8 +17,6 @@
 namespace {
 
 thread_local std::string g_error;
-hipStream_t g_stream = nullptr;
-int g_device = -1;
 
 int fail(const std::string &what)
 {
@@ -41,9 +39,68 @@ int fail_hip(hipError_t e, const char *call, int line)
         if (e_ != hipSuccess) return fail_hip(e_, #call, __LINE__); \
     } while (0)
 
-#define PS_NEED_INIT()                                                        \
-    do {                                                                      \
-        if (g_device < 0) return fail("ps_init() has not been called");       \
+// Optional per-kernel timing: an event pair around each sampled launch,
+// resolved after the run (ps_run_stats in the header).
+struct Timer {
+    ps_run_stats *stats = nullptr;
+    std::vector<hipEvent_t> ev;   // 2 per timed launch
+    std::vector<int> kind;
+    size_t used = 0;
+    long long seen[6] = {0, 0, 0, 0, 0, 0};  // launches of each kind in this run call
+};
+
+// A device context: its device, its work stream and its launch timer.  The
+// process context (ps_init) serves every thread that has no context of its
+// own current; ps_context_push makes one current on the calling thread.
+struct Ctx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    Timer timer;
+    double *sum_part = nullptr;  // partial sums of ps_level_sums (allocated on first use)
+};
+
+Ctx *g_process = nullptr;
+thread_local std::vector<Ctx *> t_stack;
+
+Ctx *current() { return t_stack.empty() ? g_process : t_stack.back(); }
+hipStream_t stream() { return current()->stream; }
+
+int create_ctx(int device, Ctx **out)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail("no HIP device is visible");
+    if (device < 0 || device >= n) return fail("device ordinal out of range");
+    PS_HIP(hipSetDevice(device));
+    Ctx *c = new Ctx;
+    c->device = device;
+    const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return fail_hip(e, "hipStreamCreateWithFlags", __LINE__);
+    }
+    *out = c;
+    return 0;
+}
+
+// Drain and release a context's stream and events.
+int destroy_ctx(Ctx *c)
+{
+    int rc = 0;
+    if (c->stream) {
+        hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = fail_hip(e, "hipStreamSynchronize", __LINE__);
+        e = hipStreamDestroy(c->stream);
+        if (e != hipSuccess && !rc) rc = fail_hip(e, "hipStreamDestroy", __LINE__);
+    }
+    for (hipEvent_t ev : c->timer.ev) (void)hipEventDestroy(ev);
+    if (c->sum_part) (void)hipFree(c->sum_part);
+    delete c;
+    return rc;
+}
+
+#define PS_NEED_INIT()                                                                            \
+    do {                                                                                          \
+        if (!current()) return fail("ps_init() has not been called and no context is current"); \
     } while (0)
 
 char *dup_host(const std::string &s)
@@ -55,25 +112,14 @@ char *dup_host(const std::string &s)
 
 int launch(ps_function f, unsigned gx, unsigned threads, void **args)
 {
-    PS_HIP(hipModuleLaunchKernel(static_cast<hipFunction_t>(f), gx, 1, 1, threads, 1, 1, 0, g_stream,
+    PS_HIP(hipModuleLaunchKernel(static_cast<hipFunction_t>(f), gx, 1, 1, threads, 1, 1, 0, stream(),
                                  args, nullptr));
     return 0;
 }
 
-// Optional per-kernel timing: an event pair around each sampled launch,
-// resolved after the run (ps_run_stats in the header).
-struct Timer {
-    ps_run_stats *stats = nullptr;
-    std::vector<hipEvent_t> ev;   // 2 per timed launch
-    std::vector<int> kind;
-    size_t used = 0;
-    long long seen[6] = {0, 0, 0, 0, 0, 0};  // launches of each kind in this run call
-};
-
-Timer g_timer;
-
 int timer_begin(ps_run_stats *stats)
 {
+    Timer &g_timer = current()->timer;
     g_timer.stats = stats;
     g_timer.used = 0;
     g_timer.kind.clear();
@@ -84,6 +130,7 @@ int timer_begin(ps_run_stats *stats)
 // whether this launch of `kind` is one the caller asked to time
 bool timer_wants(int kind)
 {
+    Timer &g_timer = current()->timer;
     const ps_run_stats *st = g_timer.stats;
     if (!st) return false;
     const long long n = g_timer.seen[kind]++;
@@ -93,6 +140,7 @@ bool timer_wants(int kind)
 
 int timer_mark(bool start, int kind)
 {
+    Timer &g_timer = current()->timer;
     if (start) {
         if (g_timer.used + 2 > g_timer.ev.size()) {
             for (int i = 0; i < 64; ++i) {
@@ -103,16 +151,17 @@ int timer_mark(bool start, int kind)
         }
         g_timer.kind.push_back(kind);
     }
-    PS_HIP(hipEventRecord(g_timer.ev[g_timer.used++], g_stream));
+    PS_HIP(hipEventRecord(g_timer.ev[g_timer.used++], stream()));
     return 0;
 }
 
 int timer_end()
 {
+    Timer &g_timer = current()->timer;
     ps_run_stats *st = g_timer.stats;
     g_timer.stats = nullptr;
     if (!st) return 0;
-    PS_HIP(hipStreamSynchronize(g_stream));
+    PS_HIP(hipStreamSynchronize(stream()));
     for (size_t i = 0; i < g_timer.kind.size(); ++i) {
         float ms = 0.f;
         PS_HIP(hipEventElapsedTime(&ms, g_timer.ev[2 * i], g_timer.ev[2 * i + 1]));
@@ -157,11 +206,98 @@ int launch_phase(const ps_swept_plan *p, ps_function f, int gts, int A0, int xof
                        kind);
 }
 
+// ---- FP64 sums of a level's variables (conserved-quantity checks) ----
+// The grid shape is fixed, so the order of the additions, and with it the
+// result, does not depend on the device or the call.
+constexpr int PS_SUM_BLOCKS = 512;
+constexpr int PS_SUM_THREADS = 256;
+constexpr int PS_SUM_MAX_NV = 8;
+constexpr size_t PS_SUM_PARTIALS = 2 * PS_SUM_MAX_NV * PS_SUM_BLOCKS;  // doubles
+
+__device__ __forceinline__ void ps_sum_tree(double *red, double &a, double &b)
+{
+    // pairwise over the workgroup; red holds 2 * PS_SUM_THREADS doubles
+    red[threadIdx.x] = a;
+    red[PS_SUM_THREADS + threadIdx.x] = b;
+    __syncthreads();
+    for (int w = PS_SUM_THREADS / 2; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            red[threadIdx.x] += red[threadIdx.x + w];
+            red[PS_SUM_THREADS + threadIdx.x] += red[PS_SUM_THREADS + threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    a = red[0];
+    b = red[PS_SUM_THREADS];
+}
+
+// Block (i, v): the i-th of PS_SUM_BLOCKS contiguous chunks of variable v;
+// part[v][i] = sum, part[nv + v][i] = sum of magnitudes, both in FP64.
+template <class R>
+__global__ void __launch_bounds__(PS_SUM_THREADS)
+    ps_sum_partial(const R *__restrict__ lvl, size_t cells, double *__restrict__ part)
+{
+    __shared__ double red[2 * PS_SUM_THREADS];
+    const int v = blockIdx.y, nv = gridDim.y;
+    const R *__restrict__ q = lvl + v * cells;
+    const size_t chunk = (cells + PS_SUM_BLOCKS - 1) / PS_SUM_BLOCKS;
+    const size_t lo = blockIdx.x * chunk;
+    const size_t hi = lo + chunk < cells ? lo + chunk : cells;
+    double s = 0.0, m = 0.0;
+    for (size_t i = lo + threadIdx.x; i < hi; i += PS_SUM_THREADS) {
+        const double x = static_cast<double>(q[i]);
+        s += x;
+        m += fabs(x);
+    }
+    ps_sum_tree(red, s, m);
+    if (threadIdx.x == 0) {
+        part[v * PS_SUM_BLOCKS + blockIdx.x] = s;
+        part[(nv + v) * PS_SUM_BLOCKS + blockIdx.x] = m;
+    }
+}
+
+// Block v: sums[v] and sums[nv + v] from the partials of variable v.
+__global__ void __launch_bounds__(PS_SUM_THREADS)
+    ps_sum_final(const double *__restrict__ part, int nv, double *__restrict__ sums)
+{
+    __shared__ double red[2 * PS_SUM_THREADS];
+    const int v = blockIdx.x;
+    double s = 0.0, m = 0.0;
+    for (int i = threadIdx.x; i < PS_SUM_BLOCKS; i += PS_SUM_THREADS) {
+        s += part[v * PS_SUM_BLOCKS + i];
+        m += part[(nv + v) * PS_SUM_BLOCKS + i];
+    }
+    ps_sum_tree(red, s, m);
+    if (threadIdx.x == 0) {
+        sums[v] = s;
+        sums[nv + v] = m;
+    }
+}
+
+int level_sums(const void *level, int fp32, int nv, size_t cells, double *sums, double *part)
+{
+    if (!level || !sums) return fail("level sums: null level or output");
+    if (nv < 1 || nv > PS_SUM_MAX_NV) return fail("level sums: nv must be 1..8");
+    (void)hipGetLastError();
+    const dim3 grid(PS_SUM_BLOCKS, static_cast<unsigned>(nv));
+    if (fp32)
+        hipLaunchKernelGGL(ps_sum_partial<float>, grid, dim3(PS_SUM_THREADS), 0, stream(),
+                           static_cast<const float *>(level), cells, part);
+    else
+        hipLaunchKernelGGL(ps_sum_partial<double>, grid, dim3(PS_SUM_THREADS), 0, stream(),
+                           static_cast<const double *>(level), cells, part);
+    PS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(ps_sum_final, dim3(static_cast<unsigned>(nv)), dim3(PS_SUM_THREADS), 0, stream(),
+                       static_cast<const double *>(part), nv, sums);
+    PS_HIP(hipGetLastError());
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
 
-int ps_version(void) { return 2; }
+int ps_version(void) { return 3; }
 
 const char *ps_last_error(void) { return g_error.c_str(); }
 
@@ -175,28 +311,77 @@ int ps_device_count(int *count)
 
 int ps_init(int device)
 {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail("no HIP device is visible");
-    if (device < 0 || device >= n) return fail("device ordinal out of range");
-    if (g_device == device && g_stream) return 0;
-    PS_HIP(hipSetDevice(device));
-    if (g_stream) {
-        PS_HIP(hipStreamDestroy(g_stream));
-        g_stream = nullptr;
+    if (g_process && g_process->device == device) return 0;
+    Ctx *c = nullptr;
+    int rc = create_ctx(device, &c);
+    if (rc) return rc;
+    if (g_process) {
+        rc = destroy_ctx(g_process);
+        g_process = nullptr;
+        if (rc) {
+            destroy_ctx(c);
+            return rc;
+        }
+        PS_HIP(hipSetDevice(device));
     }
-    PS_HIP(hipStreamCreateWithFlags(&g_stream, hipStreamNonBlocking));
-    g_device = device;
+    g_process = c;
     return 0;
 }
 
 int ps_shutdown(void)
 {
-    if (g_stream) {
-        PS_HIP(hipStreamSynchronize(g_stream));
-        PS_HIP(hipStreamDestroy(g_stream));
-    }
-    g_stream = nullptr;
-    g_device = -1;
+    if (!g_process) return 0;
+    Ctx *c = g_process;
+    g_process = nullptr;
+    return destroy_ctx(c);
+}
+
+int ps_context_create(int device, ps_context *ctx)
+{
+    if (!ctx) return fail("ps_context_create: null output");
+    *ctx = nullptr;
+    const Ctx *prev = current();
+    Ctx *c = nullptr;
+    int rc = create_ctx(device, &c);
+    // creating a context must not change the calling thread's device
+    if (prev && prev->device != device) (void)hipSetDevice(prev->device);
+    if (rc) return rc;
+    *ctx = c;
+    return 0;
+}
+
+int ps_context_destroy(ps_context ctx)
+{
+    if (!ctx) return 0;
+    Ctx *c = static_cast<Ctx *>(ctx);
+    for (Ctx *t : t_stack)
+        if (t == c) return fail("ps_context_destroy: the context is current on the calling thread");
+    if (c == g_process) return fail("ps_context_destroy: the process context is released by ps_shutdown");
+    return destroy_ctx(c);
+}
+
+int ps_context_push(ps_context ctx)
+{
+    if (!ctx) return fail("ps_context_push: null context");
+    Ctx *c = static_cast<Ctx *>(ctx);
+    PS_HIP(hipSetDevice(c->device));
+    t_stack.push_back(c);
+    return 0;
+}
+
+int ps_context_pop(void)
+{
+    if (t_stack.empty()) return fail("ps_context_pop: no context is current on this thread");
+    t_stack.pop_back();
+    const Ctx *c = current();
+    if (c) PS_HIP(hipSetDevice(c->device));
+    return 0;
+}
+
+int ps_context_current(ps_context *ctx)
+{
+    if (!ctx) return fail("ps_context_current: null output");
+    *ctx = t_stack.empty() ? nullptr : t_stack.back();
     return 0;
 }
 
@@ -241,6 +426,13 @@ int ps_compile(const char *source, const char *name, const char *const *options,
 }
 
 void ps_free_host(void *p) { std::free(p); }
+
+int ps_compiler_version(int *major, int *minor)
+{
+    const hiprtcResult r = hiprtcVersion(major, minor);
+    if (r != HIPRTC_SUCCESS) return fail(std::string("hiprtcVersion: ") + hiprtcGetErrorString(r));
+    return 0;
+}
 
 int ps_module_load(const void *code, size_t size, ps_module *module)
 {
@@ -313,24 +505,24 @@ int ps_free(void *dptr)
 int ps_memcpy_htod(void *dst, const void *src, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
-    PS_HIP(hipStreamSynchronize(g_stream));
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
+    PS_HIP(hipStreamSynchronize(stream()));
     return 0;
 }
 
 int ps_memcpy_dtoh(void *dst, const void *src, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
-    PS_HIP(hipStreamSynchronize(g_stream));
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
+    PS_HIP(hipStreamSynchronize(stream()));
     return 0;
 }
 
 int ps_memcpy_dtod(void *dst, const void *src, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
-    PS_HIP(hipStreamSynchronize(g_stream));
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
+    PS_HIP(hipStreamSynchronize(stream()));
     return 0;
 }
 
@@ -350,28 +542,28 @@ int ps_host_free(void *ptr)
 int ps_memcpy_htod_async(void *dst, const void *src, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, g_stream));
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream()));
     return 0;
 }
 
 int ps_memcpy_dtoh_async(void *dst, const void *src, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, g_stream));
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream()));
     return 0;
 }
 
 int ps_memcpy_dtod_async(void *dst, const void *src, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, g_stream));
+    PS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream()));
     return 0;
 }
 
 int ps_memset(void *dst, int value, size_t bytes)
 {
     PS_NEED_INIT();
-    PS_HIP(hipMemsetAsync(dst, value, bytes, g_stream));
+    PS_HIP(hipMemsetAsync(dst, value, bytes, stream()));
     return 0;
 }
 
@@ -380,14 +572,14 @@ int ps_launch(ps_function function, unsigned gx, unsigned gy, unsigned gz, unsig
 {
     PS_NEED_INIT();
     PS_HIP(hipModuleLaunchKernel(static_cast<hipFunction_t>(function), gx, gy, gz, bx, by, bz, shared_bytes,
-                                 g_stream, args, nullptr));
+                                 stream(), args, nullptr));
     return 0;
 }
 
 int ps_synchronize(void)
 {
     PS_NEED_INIT();
-    PS_HIP(hipStreamSynchronize(g_stream));
+    PS_HIP(hipStreamSynchronize(stream()));
     return 0;
 }
 
@@ -402,7 +594,7 @@ int ps_event_create(ps_event *event)
 int ps_event_record(ps_event event)
 {
     PS_NEED_INIT();
-    PS_HIP(hipEventRecord(static_cast<hipEvent_t>(event), g_stream));
+    PS_HIP(hipEventRecord(static_cast<hipEvent_t>(event), stream()));
     return 0;
 }
 
@@ -538,7 +730,7 @@ int ps_copy_f64(void *dst, const void *src, size_t n)
     if (blocks > 8192) blocks = 8192;
     if (blocks == 0) return 0;
     (void)hipGetLastError();
-    hipLaunchKernelGGL(ps_copy_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, g_stream,
+    hipLaunchKernelGGL(ps_copy_kernel, dim3(static_cast<unsigned>(blocks)), dim3(threads), 0, stream(),
                        static_cast<double *>(dst), static_cast<const double *>(src), n);
     PS_HIP(hipGetLastError());
     return 0;
@@ -578,19 +770,27 @@ static int multi_round(const ps_multi_plan *p)
     return launch(p->advance, static_cast<unsigned>((n + 255) / 256), 256, aargs);
 }
 
+int ps_level_sums(const void *level, int fp32, int nv, size_t cells, double *sums)
+{
+    PS_NEED_INIT();
+    Ctx *c = current();
+    if (!c->sum_part) PS_HIP(hipMalloc(&c->sum_part, PS_SUM_PARTIALS * sizeof(double)));
+    return level_sums(level, fp32, nv, cells, sums, c->sum_part);
+}
+
 int ps_multi_index(const ps_multi_plan *plan)
 {
     PS_NEED_INIT();
     if (!plan || plan->ninst <= 0) return fail("empty multi plan");
     if (plan->total_tiles > 0) {
         if (!plan->owner || !plan->prefix) return fail("multi plan without swept owner table");
-        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, g_stream,
+        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, stream(),
                            static_cast<const int *>(plan->prefix), plan->ninst, static_cast<int *>(plan->owner));
         PS_HIP(hipGetLastError());
     }
     if (plan->total_stiles > 0) {
         if (!plan->sowner || !plan->sprefix) return fail("multi plan without standard owner table");
-        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, g_stream,
+        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, stream(),
                            static_cast<const int *>(plan->sprefix), plan->ninst, static_cast<int *>(plan->sowner));
         PS_HIP(hipGetLastError());
     }
@@ -624,7 +824,7 @@ int ps_multi_run_timed(const ps_multi_plan *plan, int rounds, ps_run_stats *stat
 int ps_graph_begin(void)
 {
     PS_NEED_INIT();
-    PS_HIP(hipStreamBeginCapture(g_stream, hipStreamCaptureModeThreadLocal));
+    PS_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
     return 0;
 }
 
@@ -632,7 +832,7 @@ int ps_graph_end(ps_graph *graph)
 {
     PS_NEED_INIT();
     hipGraph_t g = nullptr;
-    PS_HIP(hipStreamEndCapture(g_stream, &g));
+    PS_HIP(hipStreamEndCapture(stream(), &g));
     hipGraphExec_t ge = nullptr;
     hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
     hipGraphDestroy(g);
@@ -644,7 +844,7 @@ int ps_graph_end(ps_graph *graph)
 int ps_graph_launch(ps_graph graph)
 {
     PS_NEED_INIT();
-    PS_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), g_stream));
+    PS_HIP(hipGraphLaunch(static_cast<hipGraphExec_t>(graph), stream()));
     return 0;
 }
 

@@ -57,8 +57,28 @@ class _Run:
         out[...] = buf
         return out
 
+    def sums(self, A):
+        """FP64 (sums, sums of magnitudes) of level A's variables, summed on
+        the device (for FP32 state too)."""
+        return hip.level_sums(self.ring.ptr + (A % self.T) * self.level_bytes, self.dtype == numpy.float32,
+                              self.nv, self.nx * self.ny)
+
     def close(self):
         self.ring.close()
+
+
+def conservation(initial, final):
+    """Conserved-sum record of a solve from the device sums of its first and
+    final levels (each a (sums, sums of magnitudes) pair): per variable the
+    change of the sum relative to the initial sum of magnitudes, and the
+    largest of those as ``drift``.  On the periodic grid both plugins
+    conserve every sum (heat content; mass, momentum and energy) up to
+    rounding, so the drift measures the rounding of the stored state."""
+    s0, m0 = initial
+    s1 = final[0]
+    rel = [float(abs(b - a) / m) if m > 0 else float(abs(b - a)) for a, b, m in zip(s0, s1, m0)]
+    return {"initial": [float(x) for x in s0], "final": [float(x) for x in s1], "relative_change": rel,
+            "drift": max(rel)}
 
 
 class StandardRun(_Run):

@@ -97,9 +97,15 @@ _version = _sig("ps_version")
 _device_count = _sig("ps_device_count", ctypes.POINTER(c_int))
 _init = _sig("ps_init", c_int)
 _shutdown = _sig("ps_shutdown")
+_ctx_create = _sig("ps_context_create", c_int, ctypes.POINTER(c_void_p))
+_ctx_destroy = _sig("ps_context_destroy", c_void_p)
+_ctx_push = _sig("ps_context_push", c_void_p)
+_ctx_pop = _sig("ps_context_pop")
+_ctx_current = _sig("ps_context_current", ctypes.POINTER(c_void_p))
 _compile = _sig("ps_compile", c_char_p, c_char_p, ctypes.POINTER(c_char_p), c_int,
                 ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p))
 _free_host = _sig("ps_free_host", c_void_p, restype=None)
+_compiler_version = _sig("ps_compiler_version", ctypes.POINTER(c_int), ctypes.POINTER(c_int))
 _module_load = _sig("ps_module_load", c_void_p, c_size_t, ctypes.POINTER(c_void_p))
 _module_unload = _sig("ps_module_unload", c_void_p)
 _get_global = _sig("ps_module_get_global", c_void_p, c_char_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t))
@@ -133,6 +139,7 @@ _swept_run_timed = _sig("ps_swept_run_timed", ctypes.POINTER(SweptPlan), c_int, 
 _standard_run_timed = _sig("ps_standard_run_timed", ctypes.POINTER(StandardPlan), c_int, c_int,
                            ctypes.POINTER(RunStats))
 _copy_f64 = _sig("ps_copy_f64", c_void_p, c_void_p, c_size_t)
+_level_sums = _sig("ps_level_sums", c_void_p, c_int, c_int, c_size_t, c_void_p)
 _multi_run = _sig("ps_multi_run", ctypes.POINTER(MultiPlan), c_int)
 _multi_index = _sig("ps_multi_index", ctypes.POINTER(MultiPlan))
 _multi_run_timed = _sig("ps_multi_run_timed", ctypes.POINTER(MultiPlan), c_int, ctypes.POINTER(RunStats))
@@ -142,7 +149,8 @@ _graph_launch = _sig("ps_graph_launch", c_void_p)
 _graph_destroy = _sig("ps_graph_destroy", c_void_p)
 
 #: every symbol the header declares (checked by the CPU test suite)
-EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shutdown", "ps_compile",
+EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shutdown", "ps_context_create",
+           "ps_context_destroy", "ps_context_push", "ps_context_pop", "ps_context_current", "ps_compile", "ps_compiler_version",
            "ps_free_host", "ps_module_load", "ps_module_unload", "ps_module_get_global",
            "ps_module_get_function", "ps_function_info", "ps_malloc", "ps_free", "ps_memcpy_htod",
            "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_host_malloc", "ps_host_free",
@@ -150,6 +158,7 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_query",
            "ps_event_destroy",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
+           "ps_level_sums",
            "ps_multi_index", "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
 
@@ -193,6 +202,64 @@ def shutdown():
     _initialised = False
 
 
+def current_context():
+    """Handle of the context current on the calling thread (``Context.handle``),
+    or None when the thread uses the process context of ``init``."""
+    h = c_void_p()
+    check(_ctx_current(ctypes.byref(h)), "ps_context_current")
+    return h.value
+
+
+class Context:
+    """A device context of its own: its device, work stream and launch timer
+    (ps_context_create).  ``with ctx:`` makes it current on the calling thread
+    (the reference's ``make_context()`` / ``push()`` / ``pop()``,
+    core/block.py:36-38); every copy, launch, run and synchronize issued by
+    that thread inside the block is ordered on this context's stream, and the
+    modules loaded inside it (``kernels.load``) are this context's own, with
+    their own ``__constant__`` memory.  One context must not be used by two
+    threads at once; distinct contexts may run on distinct threads."""
+
+    def __init__(self, device=0):
+        h = c_void_p()
+        check(_ctx_create(device, ctypes.byref(h)), "ps_context_create")
+        self.handle = h.value
+        self.device = device
+
+    def push(self):
+        check(_ctx_push(self.handle), "ps_context_push")
+
+    def pop(self):
+        check(_ctx_pop(), "ps_context_pop")
+
+    def __enter__(self):
+        self.push()
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        self.pop()
+        return False
+
+    def close(self):
+        """Drain the stream and release the context (not while current)."""
+        if self.handle:
+            check(_ctx_destroy(self.handle), "ps_context_destroy")
+            self.handle = None
+
+
+_compiler_version_value = None
+
+
+def compiler_version():
+    """(major, minor) of the hiprtc library behind ps_compile."""
+    global _compiler_version_value
+    if _compiler_version_value is None:
+        a, b = c_int(), c_int()
+        check(_compiler_version(ctypes.byref(a), ctypes.byref(b)), "ps_compiler_version")
+        _compiler_version_value = (a.value, b.value)
+    return _compiler_version_value
+
+
 def compile_source(source, name, options):
     """hiprtc-compile source text; returns (code bytes, log text)."""
     opts = (c_char_p * len(options))(*[o.encode() for o in options])
@@ -216,6 +283,9 @@ class Module:
     def __init__(self, code):
         ensure_init()
         self._code = code  # kept alive for the runtime
+        # the problem whose constants the module's __constant__ globals hold,
+        # as recorded by a caller that re-uses them (None: unknown)
+        self.const_owner = None
         h = c_void_p()
         check(_module_load(code, len(code), ctypes.byref(h)), "ps_module_load")
         self.handle = h.value
@@ -382,6 +452,23 @@ def multi_run(plan, rounds, stats=None):
 def copy_f64(dst, src, n):
     """Stream-ordered device copy of n doubles by the library's copy kernel."""
     check(_copy_f64(dst, src, n), "ps_copy_f64")
+
+
+def level_sums_async(level_ptr, fp32, nv, cells, out_ptr):
+    """Stream-ordered FP64 sums of a level's nv variables into device memory
+    ``out_ptr`` (2*nv doubles: the sums, then the sums of magnitudes)."""
+    check(_level_sums(level_ptr, int(bool(fp32)), nv, cells, out_ptr), "ps_level_sums")
+
+
+def level_sums(level_ptr, fp32, nv, cells):
+    """FP64 (sums, sums of magnitudes) of a device level [nv][cells], each an
+    array of nv values (waits for the work stream)."""
+    out = DeviceArray(2 * nv * 8)
+    level_sums_async(level_ptr, fp32, nv, cells, out.ptr)
+    host = numpy.empty(2 * nv)
+    out.download(host)
+    out.close()
+    return host[:nv].copy(), host[nv:].copy()
 
 
 class Event:

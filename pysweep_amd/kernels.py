@@ -290,12 +290,16 @@ _loaded = {}
 
 
 def load(spec):
-    """Load (once per process) the module for ``spec``; returns hip.Module."""
+    """Load (once per context) the module for ``spec``; returns hip.Module.
+    A thread with a ``hip.Context`` current gets that context's own copy, so
+    solves on distinct contexts never share a module's ``__constant__``
+    globals."""
     from pysweep_amd import hip
-    m = _loaded.get(spec)
+    key = (spec, hip.current_context())
+    m = _loaded.get(key)
     if m is None:
         m = hip.Module(build_module_code(spec))
-        _loaded[spec] = m
+        _loaded[key] = m
     return m
 
 

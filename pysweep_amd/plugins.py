@@ -109,6 +109,10 @@ class SourceModuleFacade:
 
     def get_global(self, name):
         copies = [m.get_global(name) for m in self.modules]
+        for m in self.modules:
+            # a plugin writes its constants through this handle: whoever
+            # recorded the module's constants as theirs no longer owns them
+            m.const_owner = None
         h = GlobalHandle(name, copies)
         return h, h.nbytes
 

@@ -207,6 +207,7 @@ class Solver(object):
 
     def _setup_device(self):
         nv, nx, ny = self.initialConditions.shape
+        self._sums0 = None
         t = dict(self.tuning)
         if self.simulation and self.share < 1:
             # CPU/GPU split (core/process.py:81-176): block rows divided by share
@@ -236,6 +237,8 @@ class Solver(object):
         self.cpu.set_globals(*self.globals)
         self._run.set_globals(self.cpu, self.globals)
         self._run.upload_ic(self.initialConditions)
+        # FP64 sums of the conserved quantities at the start (device-side)
+        self._sums0 = self._run.sums(self.intermediate - 1)
 
     def _collector(self):
         s = self.intermediate
@@ -260,6 +263,11 @@ class Solver(object):
         self.result_step = (last_full - (s - 1)) // s
         self.result_level = self.final_level
         self.final_state = self._run.level(self.final_level)
+        # the final level's conserved sums against the initial ones (GPU-only runs)
+        self.conservation = None
+        if getattr(self, "_sums0", None) is not None:
+            from pysweep_amd import device
+            self.conservation = device.conservation(self._sums0, self._run.sums(self.final_level))
         full_steps = self.stages / s
         nv, nx, ny = self.globalShape
         self.gcups = nx * ny * full_steps / self.solve_seconds / 1e9 if self.solve_seconds else None

@@ -54,6 +54,7 @@ def test_batched_mix_matches_oracle_bitwise(device):
         steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
         want = oracle_stages(step, p.ic, p.its, p.its * steps)
         assert np.array_equal(t.result, want), (p.plugin, p.ic.shape)
+        assert t.conservation["drift"] < 1e-13, t.conservation
 
 
 def test_alone_equals_in_mix(device):
@@ -62,6 +63,52 @@ def test_alone_equals_in_mix(device):
     for i in (0, 4):
         alone, _ = _run([mix[i][0]])
         assert np.array_equal(alone[0].result, tickets[i].result)
+
+
+def test_fp32_alone_equals_in_mix(device):
+    """FP32-state problems in a mix of FP64 and FP32 ones of both plugins,
+    with swept and standard methods: each FP32 result equals the same
+    problem's result run alone, bitwise."""
+    mix = _mix() + [_euler_problem(96, 64, 7, eps=2.0, precision="fp32"),
+                    _euler_problem(64, 64, 5, precision="fp32"),
+                    _heat_problem(64, 128, 21, kx=3, precision="fp32"),
+                    _heat_problem(128, 64, 15, kx=2, precision="fp32")]
+    mix[-1][0].method = "standard"
+    mix[-3][0].method = "standard"
+    tickets, _ = _run([p for p, _ in mix])
+    for i in range(len(mix) - 4, len(mix)):
+        alone, _ = _run([mix[i][0]])
+        r = alone[0].result  # FP32 state, returned as float64
+        assert np.array_equal(r.astype(np.float32).astype(np.float64), r)
+        assert np.array_equal(alone[0].result, tickets[i].result), i
+
+
+def test_solo_constants_follow_the_problem_across_classes(device):
+    """Two heat problems of different classes (b=8 and b=16) and different
+    time steps whose solo runs share one standard module: the first runs
+    part of its stages alone, the second runs to the end, then the first
+    resumes.  The module's constants must be the first problem's again."""
+    from pysweep_amd.batch import Engine, Problem
+    p1, step1 = _heat_problem(64, 64, 40, b=8)
+    p1.method = "standard"
+    n, steps = 64, 30
+    dx = 1.0 / n
+    dt = 0.05 * dx * dx
+    x = np.linspace(0, 1, n, endpoint=False)
+    ic2 = np.outer(np.sin(2 * np.pi * 3 * x), np.sin(2 * np.pi * x))[None]
+    p2 = Problem("heat", ic2, [0.0, steps * dt, dt, dx, dx, 1.0, True], 1, b=16, method="standard")
+    c2 = o_heat.courant_py(1.0, dt, dx)
+    step2 = lambda q, qb, gg: o_heat.step_py(q[0], None if qb is None else qb[0], gg, c2, True)[None]
+    a = Engine(max_rounds_ahead=5)
+    t1 = a.submit(p1, arrival=0.0)
+    a.step()
+    assert 0 < t1.c < t1.c_end
+    b = Engine()
+    t2 = b.submit(p2, arrival=0.0)
+    b.run()
+    a.run()
+    assert np.array_equal(t2.result, oracle_stages(step2, p2.ic, 1, steps))
+    assert np.array_equal(t1.result, oracle_stages(step1, p1.ic, 1, 40))
 
 
 @pytest.mark.parametrize("rounds", [1, 3])
@@ -182,6 +229,15 @@ def test_fp32_state_error_against_fp64_oracle(device):
     assert lin_h <= 1e-6 and l2_h <= 1e-6
     assert np.isfinite(r[1].result).all()
     assert l2_e <= 1e-3
+    # conserved sums in FP64 on the device: FP32 storage rounding moves them
+    # by about one FP32 ulp per stage at most
+    for t in r:
+        c = t.conservation
+        s0 = t.problem.ic.astype(np.float32).astype(np.float64).sum(axis=(1, 2))
+        assert np.allclose(c["initial"], s0, rtol=1e-12)
+        assert np.allclose(c["final"], t.result.sum(axis=(1, 2)), rtol=1e-12, atol=1e-9)
+        print(t.problem.plugin, "fp32 conserved-sum drift", c["drift"])
+        assert c["drift"] < 1e-5
 
 
 def test_auto_method_and_round_batching_match_oracle(device):
@@ -200,7 +256,7 @@ def test_auto_method_and_round_batching_match_oracle(device):
 
 
 def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(device, tmp_path):
-    """Tuner.tune times each candidate (method, block size, persistent grid)
+    """Tuner.tune times each candidate (method, block size, LDS window)
     on a sample batch, stores the fastest, and every candidate gives the
     same bits."""
     import json
@@ -228,6 +284,28 @@ def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(dev
         for tag, r in t["results"].items():
             assert np.array_equal(r, first[tag]), (t, tag)
     assert Tuner(path=path).method("heat", "fp64", 1) == rec["method"]
+
+
+def test_tune_classes_feeds_the_default_engine(device, tmp_path):
+    """Tuner.tune_classes tunes every class of a mix on a sample of its own
+    problems, records the chosen occupancy, and an engine built on the
+    stored records runs method="auto" problems to the oracle's bits."""
+    from pysweep_amd.autotune import Tuner
+    path = str(tmp_path / "tuning.json")
+    mix = _mix()
+    recs = Tuner(path=path).tune_classes([p for p, _ in mix], sample=3, steps=6)
+    assert set(recs) == {"euler/fp64/its2", "heat/fp64/its1", "heat/fp64/its2"}
+    for r in recs.values():
+        assert r["method"] in ("swept", "standard") and r["workgroups_per_cu"] >= 1
+        assert len(r["trials"]) >= 3
+    for p, _ in mix:
+        p.method = "auto"
+    tickets, eng = _run([p for p, _ in mix], tuner=Tuner(path=path))
+    for t, (p, step) in zip(tickets, mix):
+        rec = recs[f"{p.plugin}/fp64/its{p.its}"]
+        assert t.method == rec["method"]
+        steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
+        assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps))
 
 
 @pytest.mark.parametrize("method", ["swept", "standard"])

@@ -426,3 +426,39 @@ def test_staging_ring_first_in_first_out():
         owner[off:off + m] = i
         live.append((off, m))
     assert [tuple(x) for x in r.live] == live
+
+
+def test_context_api_without_a_current_context():
+    """The per-thread context stack: nothing is current on a fresh thread, a
+    pop with nothing pushed is an error with a message, and (without a GPU)
+    creating a context reports that no device is visible."""
+    import threading
+
+    from pysweep_amd import hip
+    seen = {}
+
+    def probe():
+        seen["current"] = hip.current_context()
+        try:
+            hip.check(hip._ctx_pop(), "ps_context_pop")
+        except hip.HipError as e:
+            seen["pop"] = str(e)
+
+    t = threading.Thread(target=probe)
+    t.start()
+    t.join()
+    assert seen["current"] is None
+    assert "no context is current" in seen["pop"]
+    if hip.device_count() == 0:
+        with pytest.raises(hip.HipError, match="no HIP device"):
+            hip.Context(0)
+
+
+def test_workgroups_per_cu_bounds():
+    """Resident workgroups per CU from LDS bytes, VGPRs and lanes (MI355X:
+    160 KB of LDS, 4 SIMDs of 8 waves and 512 VGPRs per lane)."""
+    from pysweep_amd.autotune import workgroups_per_cu
+    assert workgroups_per_cu(28 * 1024, 64, 256) == 5       # LDS-bound
+    assert workgroups_per_cu(8 * 1024, 128, 256) == 4       # 4 waves per SIMD by VGPRs
+    assert workgroups_per_cu(8 * 1024, 60, 512) == 4        # 32 waves per CU, 8 per workgroup
+    assert workgroups_per_cu(130 * 1024, 40, 512) == 1
