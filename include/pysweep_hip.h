@@ -283,6 +283,16 @@ int ps_copy_f64(void *dst, const void *src, size_t n);
  * counterpart in the reference, which checks no conserved quantity. */
 int ps_level_sums(const void *level, int fp32, int nv, size_t cells, double *sums);
 
+/* ps_level_sums without any allocation: the partial sums go to a fixed
+ * device array of the library (2 x 8 variables x 512 blocks of doubles,
+ * 64 KiB per GPU, held until the process ends) instead of a buffer the
+ * context allocates; nv above 8 is an error.  Same results, bit for bit.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops).  All
+ * calls on one GPU share the array: calls that overlap on one GPU (two
+ * threads, or two contexts' streams) race. */
+int ps_level_sums_noalloc(const void *level, int fp32, int nv, size_t cells, double *sums);
+
 /* ---------------------------------------------------------------------- */
 /* Graph capture of the work stream (no counterpart in the reference).     */
 /* ---------------------------------------------------------------------- */

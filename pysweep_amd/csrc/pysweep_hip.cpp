@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -62,6 +63,8 @@ struct Ctx {
 Ctx *g_process = nullptr;
 thread_local std::vector<Ctx *> t_stack;
 
+int scratch_setup(int device);
+
 Ctx *current() { return t_stack.empty() ? g_process : t_stack.back(); }
 hipStream_t stream() { return current()->stream; }
 
@@ -71,6 +74,7 @@ int create_ctx(int device, Ctx **out)
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail("no HIP device is visible");
     if (device < 0 || device >= n) return fail("device ordinal out of range");
     PS_HIP(hipSetDevice(device));
+    if (int rc = scratch_setup(device)) return rc;
     Ctx *c = new Ctx;
     c->device = device;
     const hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
@@ -272,6 +276,31 @@ __global__ void __launch_bounds__(PS_SUM_THREADS)
         sums[v] = s;
         sums[nv + v] = m;
     }
+}
+
+// Scratch of ps_level_sums_noalloc: the partial sums in a file-scope device
+// array instead of a buffer the context allocates.  Every GPU the library
+// runs on holds its own copy until the process ends.  In host code the
+// array's name is not a device address: the address of each device's copy
+// comes from hipGetSymbolAddress, taken once per device (with that device
+// current) when the first context on it is made, and only that address is
+// handed to kernels.  Every block of ps_sum_partial writes all of its
+// partial sums, so nothing in the array needs to start at zero.
+static __device__ __align__(256) double scratch[PS_SUM_PARTIALS];
+constexpr int PS_MAX_DEVICES = 64;
+double *g_scratch[PS_MAX_DEVICES] = {};
+std::mutex g_scratch_mutex;
+
+// Address of `device`'s copy of the scratch array; `device` is current.
+int scratch_setup(int device)
+{
+    if (device < 0 || device >= PS_MAX_DEVICES) return fail("device ordinal beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch[device]) return 0;
+    void *p = nullptr;
+    PS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[device] = static_cast<double *>(p);
+    return 0;
 }
 
 int level_sums(const void *level, int fp32, int nv, size_t cells, double *sums, double *part)
@@ -776,6 +805,17 @@ int ps_level_sums(const void *level, int fp32, int nv, size_t cells, double *sum
     Ctx *c = current();
     if (!c->sum_part) PS_HIP(hipMalloc(&c->sum_part, PS_SUM_PARTIALS * sizeof(double)));
     return level_sums(level, fp32, nv, cells, sums, c->sum_part);
+}
+
+int ps_level_sums_noalloc(const void *level, int fp32, int nv, size_t cells, double *sums)
+{
+    PS_NEED_INIT();
+    if (nv < 1 || nv > PS_SUM_MAX_NV)
+        return fail("ps_level_sums_noalloc: nv must be 1..8 (its scratch holds the partial sums of 8 variables)");
+    const int dev = current()->device;
+    double *part = dev >= 0 && dev < PS_MAX_DEVICES ? g_scratch[dev] : nullptr;
+    if (!part) return fail("ps_level_sums_noalloc: no scratch address for the current device");
+    return level_sums(level, fp32, nv, cells, sums, part);
 }
 
 int ps_multi_index(const ps_multi_plan *plan)

@@ -140,6 +140,7 @@ _standard_run_timed = _sig("ps_standard_run_timed", ctypes.POINTER(StandardPlan)
                            ctypes.POINTER(RunStats))
 _copy_f64 = _sig("ps_copy_f64", c_void_p, c_void_p, c_size_t)
 _level_sums = _sig("ps_level_sums", c_void_p, c_int, c_int, c_size_t, c_void_p)
+_level_sums_noalloc = _sig("ps_level_sums_noalloc", c_void_p, c_int, c_int, c_size_t, c_void_p)
 _multi_run = _sig("ps_multi_run", ctypes.POINTER(MultiPlan), c_int)
 _multi_index = _sig("ps_multi_index", ctypes.POINTER(MultiPlan))
 _multi_run_timed = _sig("ps_multi_run_timed", ctypes.POINTER(MultiPlan), c_int, ctypes.POINTER(RunStats))
@@ -158,7 +159,7 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_query",
            "ps_event_destroy",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
-           "ps_level_sums",
+           "ps_level_sums", "ps_level_sums_noalloc",
            "ps_multi_index", "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
 
@@ -458,6 +459,12 @@ def level_sums_async(level_ptr, fp32, nv, cells, out_ptr):
     """Stream-ordered FP64 sums of a level's nv variables into device memory
     ``out_ptr`` (2*nv doubles: the sums, then the sums of magnitudes)."""
     check(_level_sums(level_ptr, int(bool(fp32)), nv, cells, out_ptr), "ps_level_sums")
+
+
+def level_sums_noalloc_async(level_ptr, fp32, nv, cells, out_ptr):
+    """``level_sums_async`` through the entry point that allocates nothing
+    (its partial sums in the library's fixed per-GPU array)."""
+    check(_level_sums_noalloc(level_ptr, int(bool(fp32)), nv, cells, out_ptr), "ps_level_sums_noalloc")
 
 
 def level_sums(level_ptr, fp32, nv, cells):
