@@ -29,16 +29,63 @@ __device__ __forceinline__ double heat_laplace(const R *u, int idx)
     return lx + ly;
 }
 
+/*
+ * Storage rounding of a stage value.  FP64 state keeps the value.  FP32 state
+ * rounds it stochastically: a step changes u by about 8*pi^2*alpha*dt*u,
+ * which at dt = 0.1*dx^2 on fine grids (8192^2: 1.2e-7*u) is about one FP32
+ * ulp, so rounding to nearest would round every step's increment the same
+ * way and the error would grow linearly with the steps.  Rounding up with
+ * probability equal to the discarded fraction is unbiased, and the error
+ * grows like a random walk instead.  The random bits are a 32-bit
+ * multiply-xorshift hash (the "lowbias32" constants of C. Wellons' hash
+ * prospector) of the FP64 value's two words and the stage, so
+ * a given value rounds the same way in every kernel, batch and run: results
+ * stay reproducible bit for bit, swept or standard, alone or batched.  The
+ * 29 discarded mantissa bits plus a random 29-bit integer, truncated, give
+ * the rounded float exactly (carries move into the exponent as they should);
+ * values outside the normal float range round to nearest.
+ */
+__device__ __forceinline__ unsigned heat_mix(unsigned x)
+{
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    return x ^ (x >> 16);
+}
+
+template <class R>
+__device__ __forceinline__ R heat_store(double x, int globalTimeStep);
+
+template <>
+__device__ __forceinline__ double heat_store<double>(double x, int)
+{
+    return x;
+}
+
+template <>
+__device__ __forceinline__ float heat_store<float>(double x, int globalTimeStep)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const int e = (int)((b >> 52) & 0x7ff);
+    if (e < 1023 - 126 || e >= 1023 + 127)
+        return (float)x;
+    constexpr unsigned long long low = (1ULL << 29) - 1;
+    const unsigned lo = (unsigned)b, hi = (unsigned)(b >> 32);
+    const unsigned r = heat_mix(lo ^ __builtin_rotateleft32(hi, 13) ^ (0x9e3779b9U * (unsigned)(globalTimeStep + 1)));
+    return (float)__longlong_as_double((long long)((b + (r & low)) & ~low));
+}
+
 template <class R>
 __device__ __forceinline__ void heat_step(R *shared_state, int idx, int globalTimeStep)
 {
     const double d = heat_laplace(shared_state, idx);
     if (SCHEME) {
-        shared_state[idx + TIMES] = (R)(d + (double)shared_state[idx]);
+        shared_state[idx + TIMES] = heat_store<R>(d + (double)shared_state[idx], globalTimeStep);
     } else if (globalTimeStep % 2 == 0) {
-        shared_state[idx + TIMES] = (R)(1.0 * d + (double)shared_state[idx - TIMES]);
+        shared_state[idx + TIMES] = heat_store<R>(1.0 * d + (double)shared_state[idx - TIMES], globalTimeStep);
     } else {
-        shared_state[idx + TIMES] = (R)(0.5 * d + (double)shared_state[idx]);
+        shared_state[idx + TIMES] = heat_store<R>(0.5 * d + (double)shared_state[idx], globalTimeStep);
     }
 }
 
@@ -80,9 +127,9 @@ template <class R, int MAXC, class Prev, class Emit>
 __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out, int x0, int x1, int y0, int y1,
                                          int globalTimeStep, double *, const Emit &emit)
 {
+    const bool fin = globalTimeStep % 2 == 0;
     const int w = y1 - y0, n = (x1 - x0) * w;
     const float rw = __builtin_amdgcn_rcpf((float)w);
-    const bool fin = globalTimeStep % 2 == 0;
     for (int t = threadIdx.x; t < n; t += PS_NT) {
         // a compile-time width (the standard tile, unrolled swept levels)
         // divides exactly by the constant; otherwise the float reciprocal
@@ -92,11 +139,11 @@ __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out,
         const double d = heat_laplace(cur, idx);
         R v[1];
         if (SCHEME)
-            v[0] = (R)(d + (double)cur[idx]);
+            v[0] = heat_store<R>(d + (double)cur[idx], globalTimeStep);
         else if (fin)
-            v[0] = (R)(1.0 * d + (double)prev[idx]);
+            v[0] = heat_store<R>(1.0 * d + (double)prev[idx], globalTimeStep);
         else
-            v[0] = (R)(0.5 * d + (double)cur[idx]);
+            v[0] = heat_store<R>(0.5 * d + (double)cur[idx], globalTimeStep);
         if (out)
             out[idx] = v[0];
         emit(x0 + i, y0 + j, v);

@@ -1,6 +1,7 @@
 /*
  * FP32-state variant of the heat plugin: float storage, double arithmetic,
- * one rounding to float per stage.  Python half: heat.py.
+ * one rounding to float per stage (stochastic, reproducible: heat_store in
+ * heat.cu).  Python half: heat.py.
  */
 #include "heat.cu"
 

@@ -440,3 +440,43 @@ def test_fp32_state_error_is_the_schemes_one_ulp_sensitivity(device):
     e32, eulp = rel(res["fp32"]), rel(res["ulp"])
     assert eulp > 1e-4          # one ulp of the input grows past 1e-4 in 60 steps
     assert 0.2 < e32 / eulp < 10.0
+
+
+def test_fp32_heat_with_sub_ulp_increments(device):
+    """FP32-state heat where each step changes u by less than one FP32 ulp
+    (512^2, dt = dx^2/640: the mode decays by 8 pi^2 alpha dt = 4.7e-8 of u
+    per step): the stochastic storage rounding keeps the error a random walk
+    (rounding to nearest would round every increment the same way, about
+    1e-5 after these 800 steps), and swept and standard FP32 solves give
+    the same bits."""
+    from pysweep_amd import device as dev, hip, plugins, schedule, workloads
+    from pysweep_amd.equations import path as eq_path
+    n, steps, b = 512, 800, 32
+    dx = 1.0 / n
+    dt = dx * dx / 640
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
+    ic = workloads.heat_mode(n, n)
+    cpu = plugins.load_cpu_module(eq_path("heat.py"))
+    cpu.set_globals(*g)
+    out = {}
+    for name, cu, real in (("fp64", "heat.cu", "double"), ("fp32", "heat_fp32.cu", "float")):
+        r = dev.StandardRun(eq_path(cu), 1, 1, 1, n, n, real=real)
+        r.set_globals(cpu, g)
+        r.upload_ic(ic)
+        r.run(1, steps + 1)
+        out[name] = r.level(steps)
+        r.close()
+    sw = dev.SweptRun(eq_path("heat_fp32.cu"), 1, 1, 1, n, n, b, real="float")
+    sw.set_globals(cpu, g)
+    sw.upload_ic(ic)
+    last = sw.run(schedule.swept_times(steps, b, 1, 1, exact=True))
+    hip.synchronize()
+    swept32 = sw.level(last)
+    sw.close()
+    ref = out["fp64"]
+    e = out["fp32"] - ref
+    l2 = float(np.sqrt((e * e).sum() / (ref * ref).sum()))
+    linf = float(np.abs(e).max() / np.abs(ref).max())
+    print(f"fp32 heat 512^2, 800 sub-ulp steps: rel L2 {l2:.2e}, Linf {linf:.2e}")
+    assert l2 < 4e-6 and linf < 1e-5
+    assert np.array_equal(swept32, out["fp32"])
