@@ -115,7 +115,7 @@ class SweptRun(_Run):
     """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True,
-                 real="double", generic=False, tile_rows=0, pad=0):
+                 real="double", generic=False, tile_rows=0, pad=0, pfd=None):
         P = b // (2 * ops) - 1
         if b % (2 * ops) or P < 1:
             raise ValueError(f"invalid blocksize {b} for operating_points {ops}")
@@ -124,7 +124,7 @@ class SweptRun(_Run):
         super().__init__(nv, its, nx, ny, 2 * P + its, real)
         self.P, self.b, self.ops = P, b, ops
         self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads, real=real,
-                                       generic=generic, pad=pad)
+                                       generic=generic, pad=pad, pfd=pfd)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         f = {n: mod.get_function(n) for n in kernels.SWEPT_KERNELS}

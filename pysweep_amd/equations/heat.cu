@@ -129,12 +129,15 @@ __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out,
 {
     const bool fin = globalTimeStep % 2 == 0;
     const int w = y1 - y0, n = (x1 - x0) * w;
-    const float rw = __builtin_amdgcn_rcpf((float)w);
-    for (int t = threadIdx.x; t < n; t += PS_NT) {
-        // a compile-time width (the standard tile, unrolled swept levels)
-        // divides exactly by the constant; otherwise the float reciprocal
-        const int i = __builtin_constant_p(w) ? t / w : heat_div(t, w, rw);
-        const int j = t - i * w;
+    // lane t starts at cell t = (i, j) and steps by PS_NT cells, i.e. by
+    // (di, dj) with a carry, so only the first cell needs a division (a
+    // compile-time width divides by the constant; otherwise the float
+    // reciprocal)
+    const int t0 = threadIdx.x;
+    int i = __builtin_constant_p(w) ? t0 / w : heat_div(t0, w, __builtin_amdgcn_rcpf((float)w));
+    int j = t0 - i * w;
+    const int di = PS_NT / w, dj = PS_NT - di * w;
+    for (int t = t0; t < n; t += PS_NT) {
         const int idx = (x0 + i) * SY + y0 + j;
         const double d = heat_laplace(cur, idx);
         R v[1];
@@ -147,5 +150,11 @@ __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out,
         if (out)
             out[idx] = v[0];
         emit(x0 + i, y0 + j, v);
+        i += di;
+        j += dj;
+        if (j >= w) {
+            j -= w;
+            ++i;
+        }
     }
 }

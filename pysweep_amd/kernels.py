@@ -59,6 +59,7 @@ class ModuleSpec:
     mpss: int = 1         # pyramid height (standard modules of a multi class)
     generic: bool = False  # force the per-cell step even if the plugin has a level update
     pad: int = 0          # swept: LDS row padding (SY = b + pad)
+    pfd: int = 1          # swept: levels of face prefetch in flight (level update)
 
     @property
     def real_bytes(self):
@@ -69,7 +70,8 @@ class ModuleSpec:
         name = os.path.basename(self.user_path)
         if self.kind == "swept":
             return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads,
-                                         real=self.real, multi=self.multi, generic=self.generic, pad=self.pad)
+                                         real=self.real, multi=self.multi, generic=self.generic, pad=self.pad,
+                                         pfd=self.pfd)
         if self.kind == "standard":
             return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
                                             mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic)
@@ -180,8 +182,19 @@ def default_slots(nv, its, b, P, real_bytes=8, scratch=0, least=None):
     return max(least, min(full, (28 * 1024 - scratch) // level))
 
 
+def default_pfd(nv, b, threads, real_bytes):
+    """Levels of face prefetch in flight for a swept tile (PS_PFD in
+    swept.inc).  Measured on MI355X, 300 steps: heat 8192^2 b=64 292 GCUPS
+    with 1 level, 321 with 2, 326 with 4, 323 with 6 (the bridges' levels,
+    each waiting on its faces, gain most: YBridge -23 %); heat b=32 and the
+    Euler vortex at 1024^2 and 2048^2 with b=16 within 2 % for 1..6 levels
+    (their faces per level are few, or their levels long enough to hide the
+    fetch)."""
+    return 4 if nv == 1 and b >= 64 else 1
+
+
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
-               multi=False, generic=False, pad=0):
+               multi=False, generic=False, pad=0, pfd=None):
     if b % (2 * ops) != 0:
         raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
     P = b // (2 * ops) - 1
@@ -217,8 +230,10 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
         # against 7.06 with 256)
         if level > 16 * 1024 or 2 * (slots * level + scratch) > LDS_MAX:
             threads = 512
+    if pfd is None:
+        pfd = default_pfd(nv, b, threads, rb)
     return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads,
-                      real=real, multi=multi, generic=generic, pad=pad)
+                      real=real, multi=multi, generic=generic, pad=pad, pfd=int(pfd))
 
 
 def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scratch=None, threads=256,

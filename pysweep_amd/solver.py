@@ -228,7 +228,7 @@ class Solver(object):
         if self.simulation:
             self._run = device.SweptRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                         self.blocksize[0], slots=t.get("slots"), threads=t.get("threads"),
-                                        tail_standard=True, generic=t.get("generic", False))
+                                        tail_standard=True, generic=t.get("generic", False), pfd=t.get("pfd"))
         else:
             self._run = device.StandardRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                            bx=t.get("bx"), by=t.get("by"), threads=t.get("threads"),

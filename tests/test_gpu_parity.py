@@ -480,3 +480,21 @@ def test_fp32_heat_with_sub_ulp_increments(device):
     print(f"fp32 heat 512^2, 800 sub-ulp steps: rel L2 {l2:.2e}, Linf {linf:.2e}")
     assert l2 < 4e-6 and linf < 1e-5
     assert np.array_equal(swept32, out["fp32"])
+
+
+@pytest.mark.parametrize("pfd", [1, 2, 4, 6])
+def test_face_prefetch_depth_heat(device, pfd):
+    """Faces fetched 1..6 levels ahead (PS_PFD) give the oracle's bits:
+    heat 256^2, b=64 (31-level phases, the rolled level loop unrolled by
+    the depth, including depths that do not divide the level count)."""
+    ic, g, step = heat_case(256, 1, 100, seed=2)
+    s = gpu_solve("heat", ic, g, 1, 1, 64, tuning={"pfd": pfd})
+    assert np.array_equal(s.final_state, oracle_stages(step, ic, 1, 100))
+
+
+@pytest.mark.parametrize("pfd", [2, 3])
+def test_face_prefetch_depth_euler(device, pfd):
+    """The same for Euler 128^2, b=16 (6-level phases, fully unrolled)."""
+    ic, g, step = euler_case(128, 20)
+    s = gpu_solve("euler", ic, g, 2, 2, 16, tuning={"pfd": pfd})
+    assert np.array_equal(s.final_state, oracle_stages(step, ic, 2, 40))
