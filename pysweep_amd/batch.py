@@ -28,6 +28,7 @@ import ctypes
 import dataclasses
 import math
 import time
+import warnings
 
 import numpy
 
@@ -165,6 +166,9 @@ class PluginClass:
         self.layout = self.spec.const_layout()
         m = kernels.load(self.spec)
         sm = kernels.load(self.sspec)
+        self.modules = [m, sm]  # every module the class has run (division-range flags)
+        from pysweep_amd import device
+        device.range_fault(self.modules, clear=True)
         self.fn = {k: m.get_function(k + "Multi") for k in kernels.SWEPT_KERNELS}
         self.fn["Standard"] = sm.get_function("StandardMulti")
         self.fn["advance"] = m.get_function("ps_advance")
@@ -187,10 +191,13 @@ class PluginClass:
         if self._solo is None:
             sw = solo_specs(self.plugin, self.precision, self.its, self.b)[0]
             msw = kernels.load(sw)
+            self.modules.append(msw)
             self._solo = (sw, msw, {n: msw.get_function(n) for n in kernels.SWEPT_KERNELS})
         sw, msw, f = self._solo
         st = solo_standard_spec(self.plugin, self.precision, self.its, nx, ny)
         mst = kernels.load(st)
+        if mst not in self.modules:
+            self.modules.append(mst)
         return sw, st, msw, mst, f, mst.get_function("Standard")
 
 
@@ -348,6 +355,7 @@ class Ticket:
     consts: int = 0      # device address of the constant block
     sums: int = 0        # device address of the FP64 level sums: first level, then final
     conservation: dict = None  # conserved-sum record of the result (device.conservation)
+    division_exact: bool = True  # False: its class met a state outside the plugin's division range
     c: int = 0
     c_end: int = 0
     final_level: int = 0
@@ -569,6 +577,7 @@ class Engine:
             from pysweep_amd import autotune
             tuner = autotune.Tuner()  # the stored records, else the measured defaults
         self.tuner = tuner
+        self.division_faults = set()  # class keys whose modules flagged a division-range fault
         self.classes = {}
         self.tables = {}
         self.queue = []
@@ -610,7 +619,26 @@ class Engine:
         if self.idle() and self._ring is not None:
             _FREE_RINGS.append(self._ring)
             self._ring = None
+        if self.idle():
+            self._check_division_range()
         return self.finished
+
+    def _check_division_range(self):
+        """Classes whose modules met a state outside the plugin's division
+        range (device.range_fault) during the run: their tickets are marked
+        ``division_exact = False`` (their last bits may differ from IEEE
+        division; solve them again with ``Solver(tuning={"exact_division":
+        True})``), a warning names them, and the flags are cleared."""
+        from pysweep_amd import device
+        for key, cls in self.classes.items():
+            if device.range_fault(cls.modules):
+                self.division_faults.add(key)
+                for t in self.finished:
+                    if t.cls is cls:
+                        t.division_exact = False
+                warnings.warn(f"class {key} met a state outside the plugin's division range: its results may "
+                              "differ from IEEE division in the last bits")
+                device.range_fault(cls.modules, clear=True)
 
     def _stage(self, nbytes):
         """Page-locked piece for copies queued before the next fence; waits

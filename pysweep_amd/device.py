@@ -14,6 +14,26 @@ import numpy
 from pysweep_amd import hip, kernels, plugins, schedule
 
 
+def range_fault(modules, clear=False):
+    """Whether a plugin that checks its division range (the Euler plugin's
+    EU_RANGE_FAULT) met a state outside it in any of ``modules`` since the
+    last clear; ``clear`` resets the flags instead (both wait for the work
+    stream).  Modules without the flag never report one."""
+    hit = False
+    for m in modules:
+        try:
+            ptr, _ = m.get_global("EU_RANGE_FAULT")
+        except hip.HipError:
+            continue
+        if clear:
+            hip.htod(ptr, numpy.zeros(1, numpy.int32))
+        else:
+            v = numpy.zeros(1, numpy.int32)
+            hip.dtoh(v, ptr)
+            hit = hit or bool(v[0])
+    return hit
+
+
 class _Run:
     def __init__(self, nv, its, nx, ny, T, real="double"):
         self.nv, self.its, self.nx, self.ny, self.T = nv, its, nx, ny, T
@@ -57,6 +77,11 @@ class _Run:
         out[...] = buf
         return out
 
+    def range_fault(self, clear=False):
+        """Whether the plugin met a state outside its division range in any
+        of the run's modules since the last clear (``range_fault``)."""
+        return range_fault(self.modules, clear)
+
     def sums(self, A):
         """FP64 (sums, sums of magnitudes) of level A's variables, summed on
         the device (for FP32 state too)."""
@@ -85,10 +110,10 @@ class StandardRun(_Run):
     """Standard decomposition: one launch of ``Standard`` per stage."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, bx=None, by=None, threads=None, T=None, real="double",
-                 generic=False):
+                 generic=False, defines=()):
         super().__init__(nv, its, nx, ny, T or its + 1, real)
         self.spec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, bx=bx, by=by, threads=threads, real=real,
-                                          generic=generic)
+                                          generic=generic, defines=defines)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         self.plan = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=self.spec.bx,
@@ -115,7 +140,7 @@ class SweptRun(_Run):
     """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True,
-                 real="double", generic=False, tile_rows=0, pad=0, pfd=None):
+                 real="double", generic=False, tile_rows=0, pad=0, pfd=None, defines=()):
         P = b // (2 * ops) - 1
         if b % (2 * ops) or P < 1:
             raise ValueError(f"invalid blocksize {b} for operating_points {ops}")
@@ -124,7 +149,7 @@ class SweptRun(_Run):
         super().__init__(nv, its, nx, ny, 2 * P + its, real)
         self.P, self.b, self.ops = P, b, ops
         self.spec = kernels.swept_spec(cu_path, nv, ops, its, b, slots=slots, threads=threads, real=real,
-                                       generic=generic, pad=pad, pfd=pfd)
+                                       generic=generic, pad=pad, pfd=pfd, defines=defines)
         mod = kernels.load(self.spec)
         self.modules = [mod]
         f = {n: mod.get_function(n) for n in kernels.SWEPT_KERNELS}
@@ -134,7 +159,7 @@ class SweptRun(_Run):
                                   xbridge=f["XBridge"], octahedron=f["Octahedron"], down=f["DownPyramid"])
         self.tail = None
         if tail_standard:
-            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, real=real, generic=generic)
+            sspec = kernels.standard_spec(cu_path, nv, ops, its, nx, ny, real=real, generic=generic, defines=defines)
             smod = kernels.load(sspec)
             self.modules.append(smod)
             self.tail = hip.StandardPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, bx=sspec.bx,

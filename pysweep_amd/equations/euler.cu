@@ -29,6 +29,46 @@ struct eu_q {
     double r, m, n, e; /* rho, rho*u, rho*v, E */
 };
 
+/*
+ * Range of states for which the fast quotients and square roots below are
+ * bitwise the IEEE ones: rho, E and the pressure in [2^-120, 2^120], each
+ * momentum 0 or of magnitude in [2^-400, 2^120].  Every quotient of the
+ * step then has operands at least 2^53 above the subnormal range and fewer
+ * than 768 binary orders apart, and every denominator but the limiter's is
+ * a positive normal number.  A state outside it sets EU_RANGE_FAULT (a
+ * module global the host clears before a solve and reads after it: the
+ * solve is then repeated with EU_EXACT_DIVISION, where every quotient and
+ * square root is the compiler's IEEE `/` and `sqrt`).
+ */
+__device__ int EU_RANGE_FAULT;
+#ifdef EU_EXACT_DIVISION
+#define EU_EXACT 1
+#else
+#define EU_EXACT 0
+#endif
+
+__device__ __forceinline__ bool eu_pos_in_range(double v) { return v >= 0x1p-120 && v <= 0x1p120; }
+
+__device__ __forceinline__ bool eu_mom_in_range(double v)
+{
+    const double a = fabs(v);
+    return a <= 0x1p120 && (a >= 0x1p-400 || a == 0.0);
+}
+
+__device__ __forceinline__ bool eu_in_range(double r, double m, double n, double e, double p)
+{
+    return eu_pos_in_range(r) && eu_pos_in_range(e) && eu_pos_in_range(p) && eu_mom_in_range(m) &&
+           eu_mom_in_range(n);
+}
+
+/* flag a state outside the range (a vector store; a benign race: every
+ * writer writes 1) */
+__device__ __forceinline__ void eu_range_check(double r, double m, double n, double e, double p)
+{
+    if (!eu_in_range(r, m, n, e, p))
+        *(volatile int *)&EU_RANGE_FAULT = 1;
+}
+
 /* state values are read into double whatever the storage type R */
 template <class R>
 __device__ __forceinline__ eu_q eu_at(const R *s, int i)
@@ -93,10 +133,23 @@ __device__ __forceinline__ eu_rcp eu_twice(const eu_rcp &q)
 
 __device__ __forceinline__ double eu_quo(double n, const eu_rcp &q)
 {
+    if (EU_EXACT)
+        return n / q.d;
     double z = n * q.y;
     const double rem = __builtin_fma(-q.d, z, n);
     z = __builtin_fma(rem, q.y, z);
     return __builtin_amdgcn_div_fixup(z, q.d, n);
+}
+
+/* n/d for a positive normal denominator and operands in the range above:
+ * the special-case fix-up of eu_quo then changes nothing, and is left out */
+__device__ __forceinline__ double eu_quo_pos(double n, const eu_rcp &q)
+{
+    if (EU_EXACT)
+        return n / q.d;
+    const double z = n * q.y;
+    const double rem = __builtin_fma(-q.d, z, n);
+    return __builtin_fma(rem, q.y, z);
 }
 
 /* sqrt(x) as the compiler's IEEE double square root forms it -- reciprocal
@@ -105,6 +158,8 @@ __device__ __forceinline__ double eu_quo(double n, const eu_rcp &q)
  * nothing for larger x (densities and Roe sound speeds never come near) */
 __device__ __forceinline__ double eu_sqrt(double x)
 {
+    if (EU_EXACT)
+        return sqrt(x);
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y, h = y * 0.5;
     const double r = __builtin_fma(-h, g, 0.5);
@@ -121,7 +176,7 @@ __device__ __forceinline__ double eu_sqrt(double x)
  * of 2*rho */
 __device__ __forceinline__ double eu_pressure(const eu_q &q, const eu_rcp &r2)
 {
-    return GAM_M1 * (q.e - eu_quo(q.m * q.m + q.n * q.n, r2));
+    return GAM_M1 * (q.e - eu_quo_pos(q.m * q.m + q.n * q.n, r2));
 }
 
 __device__ __forceinline__ double eu_pressure(const eu_q &q)
@@ -153,14 +208,14 @@ __device__ __forceinline__ eu_q eu_flux(const eu_q &L, const eu_q &R, const eu_r
     eu_q f;
     if (XDIR) {
         f.r = L.m + R.m;
-        f.m = eu_quo(L.m * L.m, rl) + pl + eu_quo(R.m * R.m, rr) + pr;
-        f.n = eu_quo(L.m * L.n, rl) + eu_quo(R.m * R.n, rr);
-        f.e = eu_quo((L.e + pl) * L.m, rl) + eu_quo((R.e + pr) * R.m, rr);
+        f.m = eu_quo_pos(L.m * L.m, rl) + pl + eu_quo_pos(R.m * R.m, rr) + pr;
+        f.n = eu_quo_pos(L.m * L.n, rl) + eu_quo_pos(R.m * R.n, rr);
+        f.e = eu_quo_pos((L.e + pl) * L.m, rl) + eu_quo_pos((R.e + pr) * R.m, rr);
     } else {
         f.r = L.n + R.n;
-        f.m = eu_quo(L.m * L.n, rl) + eu_quo(R.m * R.n, rr);
-        f.n = eu_quo(L.n * L.n, rl) + pl + eu_quo(R.n * R.n, rr) + pr;
-        f.e = eu_quo((L.e + pl) * L.n, rl) + eu_quo((R.e + pr) * R.n, rr);
+        f.m = eu_quo_pos(L.m * L.n, rl) + eu_quo_pos(R.m * R.n, rr);
+        f.n = eu_quo_pos(L.n * L.n, rl) + pl + eu_quo_pos(R.n * R.n, rr) + pr;
+        f.e = eu_quo_pos((L.e + pl) * L.n, rl) + eu_quo_pos((R.e + pr) * R.n, rr);
     }
     return f;
 }
@@ -171,18 +226,18 @@ template <bool XDIR>
 __device__ __forceinline__ eu_q eu_dissipation(const eu_q &L, const eu_q &R, const eu_rcp &rl, const eu_rcp &rr)
 {
     const double sl = eu_sqrt(L.r), sr = eu_sqrt(R.r);
-    const double den = eu_quo(1.0, eu_recip(sl + sr));
+    const double den = eu_quo_pos(1.0, eu_recip(sl + sr));
     const double a0 = sl * sr;
-    const double a1 = (sl * eu_quo(L.m, rl) + sr * eu_quo(R.m, rr)) * den;
-    const double a2 = (sl * eu_quo(L.n, rl) + sr * eu_quo(R.n, rr)) * den;
-    const double a3 = (sl * eu_quo(L.e, rl) + sr * eu_quo(R.e, rr)) * den;
+    const double a1 = (sl * eu_quo_pos(L.m, rl) + sr * eu_quo_pos(R.m, rr)) * den;
+    const double a2 = (sl * eu_quo_pos(L.n, rl) + sr * eu_quo_pos(R.n, rr)) * den;
+    const double a3 = (sl * eu_quo_pos(L.e, rl) + sr * eu_quo_pos(R.e, rr)) * den;
     eu_q roe;
     roe.r = a0;
     roe.m = a0 * a1;
     roe.n = a0 * a2;
     roe.e = a0 * a3;
     const eu_rcp ra = eu_recip(a0);
-    double rad = eu_sqrt(eu_quo(GAMMA * eu_pressure(roe, eu_twice(ra)), ra)) + fabs(XDIR ? a1 : a2);
+    double rad = eu_sqrt(eu_quo_pos(GAMMA * eu_pressure(roe, eu_twice(ra)), ra)) + fabs(XDIR ? a1 : a2);
     if (isnan(rad)) rad = 0.0;
     eu_q d;
     d.r = rad * (L.r - R.r);
@@ -201,6 +256,11 @@ __device__ __forceinline__ eu_q eu_divergence(const S *s, int idx, int stride)
     const eu_q e = eu_at(s, idx + stride), ee = eu_at(s, idx + 2 * stride);
     const double pww = eu_pressure(ww), pw = eu_pressure(w), pc = eu_pressure(c);
     const double pe = eu_pressure(e), pee = eu_pressure(ee);
+    eu_range_check(ww.r, ww.m, ww.n, ww.e, pww);
+    eu_range_check(w.r, w.m, w.n, w.e, pw);
+    eu_range_check(c.r, c.m, c.n, c.e, pc);
+    eu_range_check(e.r, e.m, e.n, e.e, pe);
+    eu_range_check(ee.r, ee.m, ee.n, ee.e, pee);
     eu_q acc;
     acc.r = acc.m = acc.n = acc.e = 0.0;
 
@@ -387,7 +447,10 @@ __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out,
             const bool inx = i >= 2 && i < h + 2, iny = j >= 2 && j < w + 2;
             if (!(inx || iny)) continue;
             const int x = x0 - 2 + i, y = y0 - 2 + j;
-            P[x * SY + y] = eu_pressure(eu_at(cur, x * SY + y));
+            const eu_q q = eu_at(cur, x * SY + y);
+            const double p = eu_pressure(q);
+            P[x * SY + y] = p;
+            eu_range_check(q.r, q.m, q.n, q.e, p);
         }
     }
     __syncthreads();

@@ -401,6 +401,14 @@ def htod(dptr, arr):
     check(_htod(dptr, a.ctypes.data, a.nbytes), "ps_memcpy_htod")
 
 
+def dtoh(out, dptr):
+    """Copy out.nbytes bytes from device address dptr into the contiguous
+    numpy array ``out`` (waits for the work stream)."""
+    assert out.flags.c_contiguous
+    check(_dtoh(out.ctypes.data, dptr, out.nbytes), "ps_memcpy_dtoh")
+    return out
+
+
 def htod_bytes(dptr, data):
     buf = ctypes.create_string_buffer(bytes(data), len(data))
     check(_htod(dptr, ctypes.addressof(buf), len(data)), "ps_memcpy_htod")

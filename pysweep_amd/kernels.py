@@ -60,6 +60,7 @@ class ModuleSpec:
     generic: bool = False  # force the per-cell step even if the plugin has a level update
     pad: int = 0          # swept: LDS row padding (SY = b + pad)
     pfd: int = 1          # swept: levels of face prefetch in flight (level update)
+    defines: tuple = ()   # macros defined before the plugin source (e.g. EU_EXACT_DIVISION)
 
     @property
     def real_bytes(self):
@@ -71,10 +72,11 @@ class ModuleSpec:
         if self.kind == "swept":
             return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads,
                                          real=self.real, multi=self.multi, generic=self.generic, pad=self.pad,
-                                         pfd=self.pfd)
+                                         pfd=self.pfd, defines=self.defines)
         if self.kind == "standard":
             return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
-                                            mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic)
+                                            mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic,
+                                            defines=self.defines)
         raise ValueError(self.kind)
 
     def const_layout(self):
@@ -95,7 +97,8 @@ class ModuleSpec:
              if self.kind == "swept" else f"tile={self.bx}x{self.by}")
         return (f"{self.kind}{'-multi' if self.multi else ''}{'-generic' if self.generic else ''} "
                 f"{os.path.basename(self.user_path)} {self.real} "
-                f"nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}")
+                f"nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}"
+                + "".join(f" -D{d}" for d in self.defines))
 
 
 def has_level_update(user_path):
@@ -194,7 +197,7 @@ def default_pfd(nv, b, threads, real_bytes):
 
 
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
-               multi=False, generic=False, pad=0, pfd=None):
+               multi=False, generic=False, pad=0, pfd=None, defines=()):
     if b % (2 * ops) != 0:
         raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
     P = b // (2 * ops) - 1
@@ -233,7 +236,7 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
     if pfd is None:
         pfd = default_pfd(nv, b, threads, rb)
     return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads,
-                      real=real, multi=multi, generic=generic, pad=pad, pfd=int(pfd))
+                      real=real, multi=multi, generic=generic, pad=pad, pfd=int(pfd), defines=tuple(defines))
 
 
 def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scratch=None, threads=256,
@@ -282,7 +285,7 @@ def standard_tile(nv, ops, its, nx, ny, lds_budget=LDS_BUDGET, real_bytes=8, scr
 
 
 def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=None, lds_budget=LDS_BUDGET,
-                  real="double", multi=False, mpss=1, generic=False):
+                  real="double", multi=False, mpss=1, generic=False, defines=()):
     sc = None if generic or not level_update_in_standard(user_path) else level_scratch(user_path)
     if threads is None:
         # a level update gets 512 lanes: a 16x16 Euler tile's passes fit one
@@ -294,7 +297,7 @@ def standard_spec(user_path, nv, ops, its, nx, ny, bx=None, by=None, threads=Non
     if bx > nx or by > ny:
         raise ValueError("standard tile larger than the grid")
     return ModuleSpec("standard", os.path.abspath(user_path), nv, ops, its, bx=bx, by=by, threads=threads,
-                      real=real, multi=multi, mpss=mpss, generic=generic)
+                      real=real, multi=multi, mpss=mpss, generic=generic, defines=tuple(defines))
 
 
 def build_module_code(spec):
@@ -340,6 +343,10 @@ def shipped_module_specs():
                              (ch, 1, 1, 1), (ch, 1, 2, 2), (hf, 1, 1, 2)):
         for n in (64, 384, 1024):
             specs.append(standard_spec(path, nv, k, s, n, n))
+    # the Euler modules with IEEE division (a solve repeated after a state
+    # left the fast division's range)
+    specs.append(swept_spec(euler, 4, 2, 2, 16, defines=("EU_EXACT_DIVISION",)))
+    specs.append(standard_spec(euler, 4, 2, 2, 64, 64, defines=("EU_EXACT_DIVISION",)))
     for name, nv, k, s in (("euler_fp32.cu", 4, 2, 2), ("heat_fp32.cu", 1, 1, 1)):
         specs.append(standard_spec(eq_path(name), nv, k, s, 1024, 1024, real="float"))
         for b in (16, 32):

@@ -18,8 +18,18 @@ Additions: ``exact`` (take exactly N steps instead of the reference's
 quantization to multiples of the pyramid height), ``history`` (keep every
 full-step level, the reference's output dataset ``data``), ``result`` (final
 full-step state), ``solve_seconds`` and ``gcups``; ``tuning`` (launch shape:
-slots / threads / bx / by, and ``generic=True`` to call the plugin's per-cell
-step even when it provides a cooperative level update).
+slots / threads / bx / by / pfd, ``generic=True`` to call the plugin's
+per-cell step even when it provides a cooperative level update, and
+``exact_division=True`` to build the plugin with EU_EXACT_DIVISION);
+``conservation`` (the FP64 sums of the first and final levels).
+
+Division precondition.  The Euler plugin forms its quotients and square
+roots from refined hardware reciprocals; they are bitwise the IEEE ones for
+states with rho, E and p in [2^-120, 2^120] and each momentum 0 or of
+magnitude in [2^-400, 2^120].  The plugin flags any state outside that range
+it meets on the device (EU_RANGE_FAULT); the Solver then solves again with
+IEEE division and square root (``division_rerun`` is True afterwards and a
+warning says so), so results always follow IEEE arithmetic.
 """
 import os
 import time
@@ -78,12 +88,36 @@ class Solver(object):
         self._setup_device()
         self.moments.append(time.time())
         self._verbose(str(self))
+        self._solve()
+        self.division_rerun = False
+        if self._division_range_fault():
+            # a state outside the range where the Euler plugin's fast
+            # quotients are the IEEE ones: the same solve with IEEE division
+            warnings.warn("the state left the range in which the plugin's fast division is exact "
+                          "(EU_RANGE_FAULT); solving again with IEEE division and square root")
+            saved = dict(self.tuning)
+            self.tuning["exact_division"] = True
+            try:
+                self._setup_device()
+                self._solve()
+            finally:
+                self.tuning = saved
+            self.division_rerun = True
+        self.moments.append(time.time())
+        self._finish(self.moments[start], self.moments[stop])
+
+    def _solve(self):
         if self.simulation:
             self.sweptSolve()
         else:
             self.standardSolve()
-        self.moments.append(time.time())
-        self._finish(self.moments[start], self.moments[stop])
+
+    def _division_range_fault(self):
+        """Whether the GPU solve met a state outside the plugin's division
+        range (never for plugins without the check, nor for host shares)."""
+        if self.share < 1 or self.tuning.get("exact_division"):
+            return False
+        return self._run.range_fault()
 
     def __str__(self):
         from pysweep_amd import reporting
@@ -225,18 +259,21 @@ class Solver(object):
             return
         from pysweep_amd import device, hip
         hip.ensure_init(0)
+        defines = ("EU_EXACT_DIVISION",) if t.get("exact_division") else ()
         if self.simulation:
             self._run = device.SweptRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                         self.blocksize[0], slots=t.get("slots"), threads=t.get("threads"),
-                                        tail_standard=True, generic=t.get("generic", False), pfd=t.get("pfd"))
+                                        tail_standard=True, generic=t.get("generic", False), pfd=t.get("pfd"),
+                                        defines=defines)
         else:
             self._run = device.StandardRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                            bx=t.get("bx"), by=t.get("by"), threads=t.get("threads"),
-                                           generic=t.get("generic", False))
+                                           generic=t.get("generic", False), defines=defines)
         # CPU globals, then the plugin's device constants (core/block.py:50, 90)
         self.cpu.set_globals(*self.globals)
         self._run.set_globals(self.cpu, self.globals)
         self._run.upload_ic(self.initialConditions)
+        self._run.range_fault(clear=True)
         # FP64 sums of the conserved quantities at the start (device-side)
         self._sums0 = self._run.sums(self.intermediate - 1)
 

@@ -24,10 +24,12 @@ extern "C" __global__ void ps_div_check(const double *num, const double *den, do
         return;
     const double a = num[i], b = den[i];
     const eu_rcp r = eu_recip(b);
-    out[4 * i + 0] = a / b;
-    out[4 * i + 1] = eu_quo(a, r);
-    out[4 * i + 2] = a / (2 * b);
-    out[4 * i + 3] = eu_quo(a, eu_twice(r));
+    out[6 * i + 0] = a / b;
+    out[6 * i + 1] = eu_quo(a, r);
+    out[6 * i + 2] = a / (2 * b);
+    out[6 * i + 3] = eu_quo(a, eu_twice(r));
+    out[6 * i + 4] = eu_quo_pos(a, r);
+    out[6 * i + 5] = eu_quo_pos(a, eu_twice(r));
 }
 
 extern "C" __global__ void ps_sqrt_check(const double *x, double *out, int cnt)
@@ -58,7 +60,7 @@ def _run(num, den):
     assert den.shape == (n,)
     mod = _module()
     fn = mod.get_function("ps_div_check")
-    dn, dd, do = hip.DeviceArray(8 * n), hip.DeviceArray(8 * n), hip.DeviceArray(32 * n)
+    dn, dd, do = hip.DeviceArray(8 * n), hip.DeviceArray(8 * n), hip.DeviceArray(48 * n)
     dn.upload(num)
     dd.upload(den)
     block = 256
@@ -66,7 +68,7 @@ def _run(num, den):
     hip.launch(fn, (grid,), (block,), [ctypes.c_void_p(dn.ptr), ctypes.c_void_p(dd.ptr),
                                        ctypes.c_void_p(do.ptr), ctypes.c_int(n)])
     hip.synchronize()
-    out = do.download(np.empty((n, 4)))
+    out = do.download(np.empty((n, 6)))
     for a in (dn, dd, do):
         a.close()
     mod.unload()
@@ -166,3 +168,18 @@ def test_square_root_matches_ieee(device):
     assert covered.sum() == n - 1  # only 1e-300 lies below the range
     assert _same_bits(out[covered, 1], out[covered, 0]).all()
     assert _same_bits(out[:, 0], want).all()  # and the GPU's sqrt is IEEE
+
+
+def test_quotients_without_fixup_inside_the_guarded_range(device):
+    """eu_quo_pos (no special-case fix-up), used for the positive normal
+    denominators of the step, is bitwise the IEEE quotient for operands of
+    the magnitudes the division-range guard admits: numerators 0 or of
+    magnitude in [2^-520, 2^250], denominators in [2^-130, 2^130]."""
+    rng = np.random.default_rng(99)
+    n = 1 << 20
+    num = _random_normals(rng, n, -520, 250)
+    num[:1024] = 0.0
+    den = np.abs(_random_normals(rng, n, -130, 130))
+    out = _run(num, den)
+    assert _same_bits(out[:, 4], out[:, 0]).all()
+    assert _same_bits(out[:, 5], out[:, 2]).all()

@@ -498,3 +498,40 @@ def test_face_prefetch_depth_euler(device, pfd):
     ic, g, step = euler_case(128, 20)
     s = gpu_solve("euler", ic, g, 2, 2, 16, tuning={"pfd": pfd})
     assert np.array_equal(s.final_state, oracle_stages(step, ic, 2, 40))
+
+
+@pytest.mark.parametrize("swept", [True, False])
+def test_state_outside_the_division_range_is_solved_with_ieee_division(device, swept):
+    """An Euler state scaled by 2^-130 (rho, E and p below the range in which
+    the plugin's fast quotients are the IEEE ones) trips EU_RANGE_FAULT; the
+    Solver solves again with IEEE division and square root and gives the
+    oracle's bits.  The same problem unscaled does not trip it."""
+    import warnings
+    ic, g, step = euler_case(64, 6)
+    small = ic * 2.0 ** -130
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        s = gpu_solve("euler", small, g, 2, 2, 16, swept=swept)
+    assert s.division_rerun and any("division" in str(x.message) for x in w)
+    assert np.array_equal(s.final_state, oracle_stages(step, small, 2, 12))
+    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=swept)
+    assert not s2.division_rerun
+
+
+def test_engine_marks_tickets_of_a_class_that_left_the_division_range(device):
+    """The batched engine cannot re-run one problem of a class, so it marks
+    the class's tickets and warns; a class that stayed in range is not
+    marked."""
+    import warnings
+
+    from pysweep_amd.batch import Engine, Problem
+    ic, g, _ = euler_case(64, 4)
+    hic, hg, _ = heat_case(64, 1, 10)
+    eng = Engine()
+    with warnings.catch_warnings(record=True):
+        warnings.simplefilter("always")
+        t1 = eng.submit(Problem("euler", ic * 2.0 ** -130, g, 2), arrival=0.0)
+        t2 = eng.submit(Problem("heat", hic, hg, 1), arrival=0.0)
+        eng.run()
+    assert t1.division_exact is False and t2.division_exact is True
+    assert ("euler", "fp64", 2, t1.cls.b) in eng.division_faults
