@@ -349,7 +349,7 @@ def shipped_module_specs():
     specs.append(standard_spec(euler, 4, 2, 2, 64, 64, defines=("EU_EXACT_DIVISION",)))
     for name, nv, k, s in (("euler_fp32.cu", 4, 2, 2), ("heat_fp32.cu", 1, 1, 1)):
         specs.append(standard_spec(eq_path(name), nv, k, s, 1024, 1024, real="float"))
-        for b in (16, 32):
+        for b in ((16, 32) if nv > 1 else (16, 32, 64)):
             specs.append(swept_spec(eq_path(name), nv, k, s, b, real="float"))
     seen, out = set(), []
     for s in specs:
