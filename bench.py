@@ -36,7 +36,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_euler1024_b16.json")
 #: the kernel build the committed counter file was measured on (bumped when
 #: the swept or standard kernels change)
-KERNEL_BUILD = 1
+KERNEL_BUILD = 2
 
 
 def parse():
@@ -56,6 +56,14 @@ def parse():
     ap.add_argument("--serve", type=int, default=48, help="requests of the served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=4.0, help="served-stream arrival rate per second")
     ap.add_argument("--large", type=int, default=1, help="large single instances (heat 8192^2, Euler 2048^2)")
+    ap.add_argument("--large-cases", default="heat,euler", help="large instances to run")
+    ap.add_argument("--large-reals", default="double,float", help="state types of the large instances")
+    ap.add_argument("--large-steps", type=int, default=1000)
+    ap.add_argument("--precision-study", type=int, default=1,
+                    help="FP32 error beside the FP64 one-ulp sensitivity (with --large)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run the library's copy kernel on 32 MiB (three times): the reference transfer "
+                         "for scaling FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--generic", action="store_true", help="per-cell plugin step instead of the level update")
     ap.add_argument("--slots", type=int, default=None)
     ap.add_argument("--threads", type=int, default=None)
@@ -175,6 +183,8 @@ def cpu_baseline(ic, g, stages, workers=None):
     n = nx * ny
     steps = stages / 2
     return {"value": n * steps / sec / 1e9, "unit": "GCUPS", "cores": workers, "kind": "port",
+            "host_cpus_visible": os.cpu_count(), "cores_note": f"the GPU's share of the host: at most "
+                                                              f"{HOST_CORES_PER_GPU} cores per GPU",
             "sample": f"{stages} RK2 stages ({steps:g} steps) of the {nx}x{ny} Euler vortex with the vectorized "
                       f"numpy oracle in the reference's numpy step order, x-strips over {workers} processes, "
                       f"{sec:.2f} s",
@@ -538,12 +548,35 @@ def served_workload(n, rate, seed=1):
     return out
 
 
+def progress(msg):
+    """A progress line on stderr (stdout carries only the JSON result)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def copy_calibration(nbytes=32 << 20):
+    """The library's copy kernel over ``nbytes`` each way, three times (a
+    transfer of known size for scaling the HBM byte counters)."""
+    from pysweep_amd import hip
+    hip.init(0)
+    n = nbytes // 8
+    src, dst = hip.DeviceArray(8 * n), hip.DeviceArray(8 * n)
+    src.fill_zero()
+    hip.synchronize()
+    for _ in range(3):
+        hip.copy_f64(dst.ptr, src.ptr, n)
+    hip.synchronize()
+    print(json.dumps({"copy_calibration_bytes_each_way": 8 * n, "launches": 3}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported by this build "
               "(the x-strip multi-GPU decomposition is design only)", file=sys.stderr)
         sys.exit(2)
+    if a.copy_calibration:
+        copy_calibration()
+        return
     np, hip, ic, g, sw, st, times = setup(a.n, a.b, a.nsteps, a.generic, a.slots, a.threads)
     from pysweep_amd import schedule
     cells = a.n * a.n
@@ -611,19 +644,27 @@ def main():
         out["swept_equals_standard_bitwise"] = bool(np.array_equal(fin_sw, fin_st))
     sw.close()
     st.close()
+    progress(f"headline done: swept {gc.get('swept')} GCUPS, standard {gc.get('standard')} GCUPS")
     if a.large:
-        out["large_instances"] = large_instances()
-        out["precision_sensitivity"] = precision_sensitivity()
+        out["large_instances"] = large_instances(names=tuple(a.large_cases.split(",")), steps=a.large_steps,
+                                                 reals=tuple(a.large_reals.split(",")))
+        if a.precision_study:
+            out["precision_sensitivity"] = precision_sensitivity()
+        progress("large instances done")
     if not a.no_cpu:
         out["heat_cpu_case"] = heat_cpu_case()
+        progress("heat CPU case done")
     if a.batch:
         out["batched"] = batched_workload(a.batch, legs=tuple(a.batch_legs.split(",")))
+        progress("batched workload done")
     if a.serve:
         out["serving"] = served_workload(a.serve, a.rate)
         out["p50_latency_s"] = out["serving"]["engine"]["p50_s"]
         out["p99_latency_s"] = out["serving"]["engine"]["p99_s"]
+        progress("served stream done")
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(ic, g, a.cpu_stages)
+        progress("CPU baseline (vectorized oracle) done")
         out["cpu_baseline"]["product_host_path"] = product_host_path(a.n, a.b)
     print(json.dumps(out), flush=True)
 
