@@ -242,10 +242,18 @@ class Context:
         return False
 
     def close(self):
-        """Drain the stream and release the context (not while current)."""
+        """Drain the stream and release the context (not while current).
+        Caches keyed by the context (``kernels.load``) forget it first, so a
+        later context that happens to get the same handle starts afresh."""
         if self.handle:
             check(_ctx_destroy(self.handle), "ps_context_destroy")
+            for hook in CONTEXT_CLOSE_HOOKS:
+                hook(self.handle)
             self.handle = None
+
+
+#: functions called with a context's handle when it is closed
+CONTEXT_CLOSE_HOOKS = []
 
 
 _compiler_version_value = None

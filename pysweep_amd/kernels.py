@@ -307,12 +307,20 @@ def build_module_code(spec):
 _loaded = {}
 
 
+def _forget_context(handle):
+    """Drop (and unload) the modules loaded for a context being closed."""
+    for key in [k for k in _loaded if k[1] == handle]:
+        _loaded.pop(key).unload()
+
+
 def load(spec):
     """Load (once per context) the module for ``spec``; returns hip.Module.
     A thread with a ``hip.Context`` current gets that context's own copy, so
     solves on distinct contexts never share a module's ``__constant__``
     globals."""
     from pysweep_amd import hip
+    if _forget_context not in hip.CONTEXT_CLOSE_HOOKS:
+        hip.CONTEXT_CLOSE_HOOKS.append(_forget_context)
     key = (spec, hip.current_context())
     m = _loaded.get(key)
     if m is None:

@@ -462,3 +462,44 @@ def test_workgroups_per_cu_bounds():
     assert workgroups_per_cu(8 * 1024, 128, 256) == 4       # 4 waves per SIMD by VGPRs
     assert workgroups_per_cu(8 * 1024, 60, 512) == 4        # 32 waves per CU, 8 per workgroup
     assert workgroups_per_cu(130 * 1024, 40, 512) == 1
+
+
+def test_conservation_record():
+    """The conserved-sum record: per variable |final - initial| over the
+    initial sum of magnitudes, and their maximum as the drift."""
+    import numpy as np
+
+    from pysweep_amd.device import conservation
+    c = conservation((np.array([10.0, -2.0]), np.array([20.0, 4.0])), (np.array([10.5, -2.0]),))
+    assert c["relative_change"] == [0.025, 0.0] and c["drift"] == 0.025
+    assert c["initial"] == [10.0, -2.0] and c["final"] == [10.5, -2.0]
+    z = conservation((np.array([0.0]), np.array([0.0])), (np.array([1e-20]),))
+    assert z["drift"] == 1e-20  # an all-zero field: the absolute change
+
+
+def test_tuner_records_and_shapes(tmp_path):
+    """Tuner records: defaults without a file, launch shapes only for the
+    block size they were tuned with, the method of method="auto"
+    problems, and the candidate shapes as class_specs arguments."""
+    import json
+
+    from pysweep_amd import batch
+    from pysweep_amd.autotune import CANDIDATES, Tuner, shape_of
+    t = Tuner(path=None)
+    assert t.block_size("heat", "fp64", 1, 96, 64) == 32 and t.block_size("euler", "fp64", 2, 96, 96) == 16
+    assert t.block_size("heat", "fp64", 1, 48, 80) == 16
+    assert t.method("euler", "fp64", 2) == batch.DEFAULT_METHOD[("euler", "fp64")]
+    path = tmp_path / "tuning.json"
+    path.write_text(json.dumps({"euler/fp64/its2": {"b": 16, "method": "standard", "stile": [8, 32],
+                                                    "sthreads": 512}}))
+    t = Tuner(path=str(path))
+    assert t.config("euler", "fp64", 2, 16) == {"stile": (8, 32), "sthreads": 512}
+    assert t.config("euler", "fp64", 2, 32) == {}
+    assert t.method("euler", "fp64", 2) == "standard"
+    for plugin, its in (("euler", 2), ("heat", 1)):
+        for cand in CANDIDATES[plugin]:
+            sp, ssp = batch.class_specs(plugin, "fp64", its, cand["b"], **shape_of(cand))
+            if "stile" in cand:
+                assert (ssp.bx, ssp.by, ssp.threads) == (*cand["stile"], cand["sthreads"])
+            if "slots" in cand:
+                assert sp.slots == cand["slots"]
