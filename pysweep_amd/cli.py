@@ -24,23 +24,29 @@ import numpy
 from pysweep_amd import equations
 
 
+_TRUTH = {"yes": True, "true": True, "t": True, "y": True, "1": True,
+          "no": False, "false": False, "f": False, "n": False, "0": False}
+
+
 def str2bool(v):
-    """utils/commandline.py:12-20."""
+    """Boolean option values as the reference's CLI accepts them
+    (utils/commandline.py:12-20)."""
     if isinstance(v, bool):
         return v
-    if v.lower() in ("yes", "true", "t", "y", "1"):
-        return True
-    if v.lower() in ("no", "false", "f", "n", "0"):
-        return False
-    raise argparse.ArgumentTypeError("Boolean value expected as string.")
+    try:
+        return _TRUTH[v.lower()]
+    except KeyError:
+        raise argparse.ArgumentTypeError("Boolean value expected as string.") from None
 
 
 def adjust_array_size(args):
-    """Round nx up to a multiple of the block size (commandline.py:22-26)."""
-    rem = args.spacesteps % args.blocksize
-    if rem:
-        args.spacesteps += args.blocksize - rem
-        warnings.warn(f"Adjusting array size to fit blocksize. New array size is {args.spacesteps}.")
+    """nx grows to the next multiple of the block size, with a warning
+    (commandline.py:22-26)."""
+    b = args.blocksize
+    grown = -(-args.spacesteps // b) * b
+    if grown != args.spacesteps:
+        args.spacesteps = grown
+        warnings.warn(f"Adjusting array size to fit blocksize. New array size is {grown}.")
 
 
 def euler_vortex(npx, npy, t=0.0, gamma=1.4):

@@ -1,31 +1,38 @@
-"""Verbose banner and log.yaml records (core/io.py:10-33, 102-132)."""
+"""Verbose solve summary and log.yaml records (core/io.py:10-33)."""
 import os
 from datetime import datetime
 
+import numpy
 import yaml
 
 
 def solver_print(solver):
-    s = "\nPysweep simulation properties:\n"
-    s += "\tsimulation type: {}\n".format("swept" if solver.simulation else "standard")
-    s += "\tnumber of processes: 1\n"
-    s += "\tgpu source: {}\n".format(solver.gpuStr)
-    s += "\tcpu source: {}\n".format(solver.cpuStr)
-    s += "\toutput file: {}\n".format(solver.output)
-    s += "\tverbose: {}\n".format(solver.verbose)
-    s += "\tArray shape: {}\n".format(getattr(solver, "arrayShape", None))
-    s += "\tdtype: {}\n".format(solver.dtype)
-    s += "\tshare: {}\n".format(solver.share)
-    s += "\tblocksize: {}\n".format(solver.blocksize[0])
-    s += "\tstencil size: {}\n".format(int(solver.operating * 2 + 1))
-    s += "\tintermediate time steps: {}\n".format(solver.intermediate)
+    """The verbose summary of a solve (the reference prints its own banner,
+    core/io.py:102-132; this one is laid out for one GPU)."""
+    method = "swept" if solver.simulation else "standard"
+    rows = [("method", f"{method}, blocksize {solver.blocksize[0]}, {solver.operating} operating points "
+                       f"({2 * solver.operating + 1}-point cross), {solver.intermediate} intermediate steps"),
+            ("placement", f"share {solver.share} of the block rows on the GPU, the rest on the host"
+                          if solver.share < 1 else "whole grid on one GPU"),
+            ("state", f"{getattr(solver, 'arrayShape', None)} {_dtype_name(solver.dtype)}"),
+            ("plugins", f"{solver.cpuStr} | {solver.gpuStr}")]
     if len(solver.globals) >= 3:
-        s += "\ttime data (t0,tf,dt): ({},{},{})\n".format(*solver.globals[:3])
-    if solver.globals[5:]:
-        s += "\tglobals: [" + ",".join(str(g) for g in solver.globals) + "]\n"
+        t0, tf, dt = solver.globals[:3]
+        rows.append(("time", f"t0 {t0}, tf {tf}, dt {dt}"))
+    if solver.globals[3:]:
+        rows.append(("plugin globals", ", ".join(str(g) for g in solver.globals[3:])))
+    rows.append(("output", solver.output or "none"))
     if solver.exid:
-        s += "\texcluded gpus: [" + ",".join(str(e) for e in solver.exid) + "]\n"
-    return s
+        rows.append(("excluded devices", ", ".join(str(e) for e in solver.exid)))
+    width = max(len(k) for k, _ in rows)
+    return "\npysweep_amd solve\n" + "".join(f"  {k:<{width}}  {v}\n" for k, v in rows)
+
+
+def _dtype_name(dtype):
+    try:
+        return numpy.dtype(dtype).name
+    except TypeError:
+        return str(dtype)
 
 
 def short_entry(solver, clocktime):
