@@ -197,16 +197,17 @@ def default_pfd(nv, b, threads, real_bytes):
     return 4 if nv == 1 and b >= 64 else 1
 
 
-def default_unroll(nv):
+def default_unroll(nv, b):
     """Most levels of a swept phase whose level loop is unrolled (every
     level's rectangles and index divisors then compile-time constants).
     Measured on MI355X: Euler b=16 (6-level Octahedron) 105 -> 87 us per
     launch and b=32 (14 levels, 70 KB of code) 1053 -> 1022 us; heat 8192^2
-    b=32 (15- and 30-level phases) 180 -> 195 GCUPS, and b=64 with its
-    31-level bridges unrolled +2 %, but its 62-level Octahedron unrolled
-    (69 KB of code) 10 % slower.  The Euler level update is ten times the
-    heat one's code, so its phases stop at 14 levels."""
-    return 31 if nv == 1 else 14
+    b=32 (15- and 30-level phases, all unrolled) FP32 state 191 -> 245
+    GCUPS, FP64 183 either way; heat b=64 with its 31-level bridges unrolled
+    FP64 +1 %, FP32 -5 %, and its 62-level Octahedron unrolled (69 KB of
+    code) 10 % slower.  The Euler level update is ten times the heat one's
+    code, so its phases stop at 14 levels."""
+    return 31 if nv == 1 and b <= 32 else 14
 
 
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
@@ -249,7 +250,7 @@ def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=
     if pfd is None:
         pfd = default_pfd(nv, b, threads, rb)
     if unroll is None:
-        unroll = default_unroll(nv)
+        unroll = default_unroll(nv, b)
     return ModuleSpec("swept", os.path.abspath(user_path), nv, ops, its, b=b, slots=slots, threads=threads,
                       real=real, multi=multi, generic=generic, pad=pad, pfd=int(pfd), defines=tuple(defines),
                       unroll=int(unroll))
