@@ -545,6 +545,10 @@ class Engine:
     (17.05-17.12 solves/s against 16.97-17.10; 512-problem mix 193-195
     against 191-195).
 
+    ``poison``: a debug mode that fills each problem's ring with NaN before
+    its initial condition goes in (a read of a cell that no phase wrote then
+    turns the result into NaN).
+
     ``tuner``: the ``autotune.Tuner`` whose records give each class its
     block size, the method of problems submitted with method="auto" and the
     workgroup shapes of its kernels; "default" loads the stored records
@@ -553,7 +557,7 @@ class Engine:
 
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, tuner="default", clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
-                 capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4):
+                 capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4, poison=False):
         hip.ensure_init(0)
         self.policy = policy
         self.admit_bytes = admit_bytes  # initial-condition bytes staged per step at most
@@ -577,6 +581,7 @@ class Engine:
             from pysweep_amd import autotune
             tuner = autotune.Tuner()  # the stored records, else the measured defaults
         self.tuner = tuner
+        self.poison = poison  # debug: rings filled with NaN before each problem's initial condition
         self.division_faults = set()  # class keys whose modules flagged a division-range fault
         self.classes = {}
         self.tables = {}
@@ -904,6 +909,8 @@ class Engine:
         # one buffer: the ring, then the constant block (lvl is a multiple
         # of 256 bytes: nx and ny are multiples of b >= 8)
         t.ring = self.rings.take(need, limit=self.hbm_budget - self.in_use)
+        if self.poison:  # every level NaN until written (stream ordered, before the uploads)
+            t.ring.fill_bytes(0xFF, T * lvl)
         t.consts = t.ring.ptr + T * lvl
         t.sums = t.consts + 256
         # levels 0 .. its-1 hold the initial condition: one upload from a

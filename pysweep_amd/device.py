@@ -49,6 +49,13 @@ class _Run:
         """The plugin writes its device constants into every module."""
         cpu_module.set_globals(*globals_, source_mod=self.facade())
 
+    def poison(self):
+        """Fill every level of the ring with NaN (all bytes 0xFF, a NaN in
+        FP64 and in FP32): a debug mode in which a read of a cell that no
+        phase wrote -- a skipped private cell, a face outside the sets --
+        turns the result into NaN instead of passing unnoticed."""
+        self.ring.fill_bytes(0xFF)
+
     def upload_ic(self, ic):
         ic = numpy.ascontiguousarray(ic, dtype=self.dtype)
         if ic.shape != (self.nv, self.nx, self.ny):

@@ -344,6 +344,11 @@ class DeviceArray:
     def fill_zero(self):
         check(_memset(self.ptr, 0, self.nbytes), "ps_memset")
 
+    def fill_bytes(self, value, nbytes=None):
+        """Set the first ``nbytes`` (default all) bytes to ``value``
+        (stream ordered)."""
+        check(_memset(self.ptr, int(value) & 0xFF, self.nbytes if nbytes is None else int(nbytes)), "ps_memset")
+
     def close(self):
         if self.ptr:
             _free(self.ptr)

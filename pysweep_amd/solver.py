@@ -20,7 +20,9 @@ full-step level, the reference's output dataset ``data``), ``result`` (final
 full-step state), ``solve_seconds`` and ``gcups``; ``tuning`` (launch shape:
 slots / threads / bx / by / pfd, ``generic=True`` to call the plugin's
 per-cell step even when it provides a cooperative level update, and
-``exact_division=True`` to build the plugin with EU_EXACT_DIVISION);
+``exact_division=True`` to build the plugin with EU_EXACT_DIVISION, and
+``poison=True`` to fill the device ring with NaN before the initial
+condition goes in, so that a read of a cell no phase wrote shows up as NaN);
 ``conservation`` (the FP64 sums of the first and final levels).
 
 Division precondition.  The Euler plugin forms its quotients and square
@@ -272,6 +274,8 @@ class Solver(object):
         # CPU globals, then the plugin's device constants (core/block.py:50, 90)
         self.cpu.set_globals(*self.globals)
         self._run.set_globals(self.cpu, self.globals)
+        if t.get("poison"):
+            self._run.poison()
         self._run.upload_ic(self.initialConditions)
         self._run.range_fault(clear=True)
         # FP64 sums of the conserved quantities at the start (device-side)

@@ -535,3 +535,39 @@ def test_engine_marks_tickets_of_a_class_that_left_the_division_range(device):
         eng.run()
     assert t1.division_exact is False and t2.division_exact is True
     assert ("euler", "fp64", 2, t1.cls.b) in eng.division_faults
+
+
+@pytest.mark.parametrize("name,b,k,s,fe", [("heat", 8, 1, 1, 1), ("heat", 32, 1, 2, 0), ("euler", 16, 2, 2, None),
+                                           ("euler", 32, 2, 2, None)])
+def test_nan_poisoned_ring_swept_equals_oracle(device, name, b, k, s, fe):
+    """Debug mode: the ring is filled with NaN before the initial condition
+    goes in, so any read of a cell no phase wrote (a skipped private cell,
+    a face outside the sets) would turn the result into NaN.  The swept and
+    standard results still equal the oracle bitwise."""
+    if name == "heat":
+        ic, g, step = heat_case(128, fe, 40)
+        stages = 40 * s
+    else:
+        ic, g, step = euler_case(128, 12)
+        stages = 24
+    want = oracle_stages(step, ic, s, stages)
+    for swept in (True, False):
+        r = gpu_solve(name, ic, g, k, s, b, swept=swept, tuning={"poison": True})
+        assert np.isfinite(r.final_state).all()
+        assert np.array_equal(r.final_state, want), swept
+
+
+def test_nan_poisoned_engine_rings(device):
+    """The engine's debug mode: every ring NaN before its initial condition;
+    the batched mix (swept classes with private cells skipped) still gives
+    the oracle's bits."""
+    from pysweep_amd.batch import Engine, Problem
+    cases = [heat_case(64, 1, 30), heat_case(128, 0, 20, seed=1), euler_case(64, 9), euler_case(128, 5)]
+    probs = [Problem("heat", cases[0][0], cases[0][1], 1), Problem("heat", cases[1][0], cases[1][1], 2),
+             Problem("euler", cases[2][0], cases[2][1], 2), Problem("euler", cases[3][0], cases[3][1], 2)]
+    eng = Engine(poison=True)
+    tickets = [eng.submit(p, arrival=0.0) for p in probs]
+    eng.run()
+    for t, p, (ic, g, step) in zip(tickets, probs, cases):
+        steps = int(round((g[1] - g[0]) / g[2]))
+        assert np.array_equal(t.result, oracle_stages(step, ic, p.its, p.its * steps)), p.plugin
