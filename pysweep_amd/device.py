@@ -97,6 +97,9 @@ class _Run:
 
     def close(self):
         self.ring.close()
+        ic = getattr(self, "ic_dev", None)
+        if ic is not None:
+            ic.close()
 
 
 def conservation(initial, final):

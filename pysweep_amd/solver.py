@@ -99,6 +99,7 @@ class Solver(object):
                           "(EU_RANGE_FAULT); solving again with IEEE division and square root")
             saved = dict(self.tuning)
             self.tuning["exact_division"] = True
+            self._run.close()  # the fast build's ring; the IEEE build gets its own
             try:
                 self._setup_device()
                 self._solve()
