@@ -150,7 +150,7 @@ class SweptRun(_Run):
     """Swept decomposition: UpPyramid, bridges, Octahedron, DownPyramid."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, b, slots=None, threads=None, tail_standard=True,
-                 real="double", generic=False, tile_rows=0, pad=0, pfd=None, defines=()):
+                 real="double", generic=False, tile_rows=0, pad=None, pfd=None, defines=()):
         P = b // (2 * ops) - 1
         if b % (2 * ops) or P < 1:
             raise ValueError(f"invalid blocksize {b} for operating_points {ops}")

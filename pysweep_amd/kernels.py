@@ -210,14 +210,27 @@ def default_unroll(nv, b):
     return 31 if nv == 1 and b <= 32 else 14
 
 
+def default_pad(nv, b):
+    """Doubles of padding per LDS row of a swept tile (SY = b + pad).  Rows of
+    16 doubles put every other row on the same LDS banks, and the level
+    passes read several rows per wavefront.  Measured on MI355X (Euler
+    vortex, 100-200 steps): b=16 FP64 1024^2 9.52 GCUPS with no padding,
+    9.65 with 1, 9.75 with 2, 9.68 with 3; 2048^2 9.85 -> 9.99 with 2; FP32
+    state b=16 10.68 -> 10.79 with 2; b=32 within 0.3 % for 0..2.  Heat's
+    single-variable level update measured no gain (b=64, 324 against 330)."""
+    return 2 if nv > 1 and b == 16 else 0
+
+
 def swept_spec(user_path, nv, ops, its, b, slots=None, threads=None, lds_budget=LDS_BUDGET, real="double",
-               multi=False, generic=False, pad=0, pfd=None, defines=(), unroll=None):
+               multi=False, generic=False, pad=None, pfd=None, defines=(), unroll=None):
     if b % (2 * ops) != 0:
         raise ValueError(f"blocksize {b} must be a multiple of 2*operating_points={2 * ops}")
     P = b // (2 * ops) - 1
     if P < 1:
         raise ValueError(f"blocksize {b} too small for operating_points={ops} (needs b >= {4 * ops})")
     rb = 8 if real == "double" else 4
+    if pad is None:
+        pad = default_pad(nv, b)
     level = nv * b * (b + pad) * rb
     full = 2 * P + its
     scratch = level_scratch_bytes(user_path, generic, b, b + pad, swept_faces(b, ops))

@@ -358,7 +358,7 @@ def test_reference_euler_vortex_120_b24(device):
     assert np.abs(sol.final_state[0] - ex[0]).max() < 0.2
 
 
-@pytest.mark.parametrize("pad", [1, 2])
+@pytest.mark.parametrize("pad", [0, 1, 2, 3])
 def test_padded_lds_rows(device, pad):
     """Swept tiles whose LDS rows are padded (SY = b + pad) give the same bits."""
     from pysweep_amd import device as dev, hip, plugins, schedule
