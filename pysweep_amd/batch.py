@@ -32,7 +32,7 @@ import warnings
 
 import numpy
 
-from pysweep_amd import hip, kernels, plugins, schedule
+from pysweep_amd import device, hip, kernels, plugins, schedule
 from pysweep_amd.equations import path as eq_path
 
 PLUGINS = {
@@ -167,7 +167,6 @@ class PluginClass:
         m = kernels.load(self.spec)
         sm = kernels.load(self.sspec)
         self.modules = [m, sm]  # every module the class has run (division-range flags)
-        from pysweep_amd import device
         device.range_fault(self.modules, clear=True)
         self.fn = {k: m.get_function(k + "Multi") for k in kernels.SWEPT_KERNELS}
         self.fn["Standard"] = sm.get_function("StandardMulti")
@@ -306,7 +305,6 @@ def _fetch(item):
     """A finished ticket's result out of its staging piece (FP64 copy), and
     its conserved-sum record from the device sums of its first and final
     levels."""
-    from pysweep_amd import device
     t, (piece, dtype, shape) = item
     t.result = piece.view(dtype, shape).astype(numpy.float64)
     nv = shape[0]
@@ -634,7 +632,6 @@ class Engine:
         ``division_exact = False`` (their last bits may differ from IEEE
         division; solve them again with ``Solver(tuning={"exact_division":
         True})``), a warning names them, and the flags are cleared."""
-        from pysweep_amd import device
         for key, cls in self.classes.items():
             if device.range_fault(cls.modules):
                 self.division_faults.add(key)
