@@ -312,8 +312,17 @@ def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float
             if real not in reals:
                 continue
             runs = [("standard", None)] + [("swept", b) for b in bs]
+            if name == "heat":
+                # the swept solve of the band kernel (levels in registers,
+                # halos recomputed instead of bridged)
+                runs.append(("band", None))
             for method, b in runs:
-                if method == "swept":
+                if method == "band":
+                    r = device.BandRun(eq_path(cu), nv, k, s, n, n, real=real)
+                    times = schedule.standard_times(steps, 8, k, s)
+                    go = (lambda st, r=r, times=times: r.run(times, stats=st))
+                    last = s * steps + s - 1
+                elif method == "swept":
                     r = device.SweptRun(eq_path(cu), nv, k, s, n, n, b, real=real)
                     times = schedule.swept_times(steps, b, k, s, exact=True)
                     go = (lambda st, r=r, times=times: r.run(times, stats=st))
@@ -341,6 +350,8 @@ def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float
                 rb = 8 if real == "double" else 4
                 if kern == "Standard":
                     cs = n * n
+                elif kern == "Band":
+                    cs = n * n * r.spec.levels
                 else:
                     cs = (n // b) ** 2 * schedule.phase_cell_stages(kern, b, k, b // (2 * k) - 1)
                 flops = FLOPS_PER_CELL_STAGE if name == "euler" else HEAT_FLOPS_PER_CELL_STAGE
@@ -354,6 +365,11 @@ def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float
                        # (stage 1: 32 R + 32 W; stage 2: 64 R + 32 W), halved for FP32 state
                        "algorithmic_gbs": n * n * steps * (16 if name == "heat" else 160) * rb / 8 / sec / 1e9}
                 rec["algorithmic_hbm_frac"] = rec["algorithmic_gbs"] / HBM_PEAK_GBS
+                if method == "band":
+                    # per launch: one read and one write of every cell
+                    # (levels stages), the recomputed halo columns aside
+                    rec["band_levels"], rec["band_rows"] = r.spec.levels, r.rows
+                    rec["band_launch_hbm_gbs"] = n * n * 2 * rb / avg / 1e9
                 out.append(rec)
                 r.close()
         ref = finals.get(("double", "standard", None))

@@ -216,15 +216,66 @@ int ps_standard_run(const ps_standard_plan *plan, int gts_begin, int gts_end);
  * 1 << PS_K_*, 0 = every kind) and `every` (time one launch in `every` of
  * each timed kind, counted per run call; 0 or 1 = each) sample the
  * launches, so that timing a long run adds few event records to it. */
-enum { PS_K_UP = 0, PS_K_YBRIDGE = 1, PS_K_XBRIDGE = 2, PS_K_OCTAHEDRON = 3, PS_K_DOWN = 4, PS_K_STANDARD = 5 };
+enum {
+    PS_K_UP = 0,
+    PS_K_YBRIDGE = 1,
+    PS_K_XBRIDGE = 2,
+    PS_K_OCTAHEDRON = 3,
+    PS_K_DOWN = 4,
+    PS_K_STANDARD = 5,
+    PS_K_BAND = 6
+};
+#define PS_K_COUNT 7
 typedef struct ps_run_stats {
-    double ms[6];
-    long long launches[6];
+    double ms[PS_K_COUNT];
+    long long launches[PS_K_COUNT];
     int kinds;
     int every;
 } ps_run_stats;
 int ps_swept_run_timed(const ps_swept_plan *plan, int G, int c_begin, int c_end, ps_run_stats *stats);
 int ps_standard_run_timed(const ps_standard_plan *plan, int gts_begin, int gts_end, ps_run_stats *stats);
+
+/* Band solve: the swept solve of a single-variable plugin on a radius-1
+ * cross stencil that provides a point update (PS_HAS_POINT), e.g. heat.
+ * Each launch of the `Band` kernel advances every cell `levels` levels (the
+ * last launch fewer) with all intermediate levels in registers, one HBM read
+ * and one HBM write per cell: bands of 128 columns, each swept along x by
+ * one wavefront, whose sloped edges are recomputed from a halo of `levels`
+ * columns instead of being exchanged through bridge phases.  Kernel
+ * Band(double *state, int globalTimeStep, int level, ps_band_grid g).
+ * Replaces Solver.sweptSolve with functions.FirstPrism/UpPrism/LastPrism
+ * (core/solver.py:158-176; core/functions.py:12-88) for such plugins: the
+ * same stages on the same ring, with results bitwise those of the phase
+ * kernels and the standard kernel. */
+typedef struct ps_band_grid {
+    int nx, ny;
+    int T;          /* ring levels                                         */
+    int hh;         /* levels of this launch                               */
+    int wlo;        /* levels hh' >= wlo of the launch are written         */
+    int rows;       /* output rows per wavefront                           */
+    int bands;      /* ceil(ny / (128 - 2 * levels))                       */
+    int segs;       /* ceil(nx / rows)                                     */
+} ps_band_grid;
+
+typedef struct ps_band_plan {
+    void *state;    /* ring state[T][nx][ny]                               */
+    int nx, ny;
+    int its;        /* intermediate steps s (1 or 2)                       */
+    int T;          /* at least levels + s                                 */
+    int levels;     /* levels per launch compiled into the kernel (even,
+                       a multiple of s, 2..16)                              */
+    int rows;       /* output rows per wavefront (> 0)                     */
+    int store_all;  /* 1: write every level; 0: the last s of each launch  */
+    ps_function band;
+} ps_band_plan;
+
+/* Run stages from absolute level A (which, with the s-1 before it, must be
+ * complete in the ring and be a full-step level) for `stages` stages, in
+ * launches of plan->levels stages (the last one fewer); the last s levels
+ * reached are complete in the ring afterwards, and every level when
+ * store_all is set. */
+int ps_band_run(const ps_band_plan *plan, int A, int stages);
+int ps_band_run_timed(const ps_band_plan *plan, int A, int stages, ps_run_stats *stats);
 
 /* ---------------------------------------------------------------------- */
 /* Instance tables: many problems of one plugin class per launch.          */

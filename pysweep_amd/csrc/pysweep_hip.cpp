@@ -47,7 +47,7 @@ struct Timer {
     std::vector<hipEvent_t> ev;   // 2 per timed launch
     std::vector<int> kind;
     size_t used = 0;
-    long long seen[6] = {0, 0, 0, 0, 0, 0};  // launches of each kind in this run call
+    long long seen[PS_K_COUNT] = {};  // launches of each kind in this run call
 };
 
 // A device context: its device, its work stream and its launch timer.  The
@@ -326,7 +326,7 @@ int level_sums(const void *level, int fp32, int nv, size_t cells, double *sums, 
 
 extern "C" {
 
-int ps_version(void) { return 3; }
+int ps_version(void) { return 4; }
 
 const char *ps_last_error(void) { return g_error.c_str(); }
 
@@ -747,6 +747,55 @@ int ps_standard_run_timed(const ps_standard_plan *p, int gts_begin, int gts_end,
     PS_NEED_INIT();
     timer_begin(stats);
     int rc = standard_run(p, gts_begin, gts_end);
+    int rc2 = timer_end();
+    return rc ? rc : rc2;
+}
+
+static int band_run(const ps_band_plan *p, int A, int stages)
+{
+    if (!p || !p->state || !p->band) return fail("incomplete band plan");
+    const int H = p->levels, s = p->its;
+    if (s < 1 || s > 2) return fail("band solve: intermediate steps must be 1 or 2");
+    if (H < 2 || H > 16 || H % 2 || H % s) return fail("band solve: levels must be even, a multiple of s, 2..16");
+    if (p->T < H + s) return fail("band ring must hold at least levels + s levels");
+    if (p->nx < 1 || p->ny < 1 || p->rows < 1) return fail("band solve: empty grid or row segment");
+    if (A < s - 1 || (A - (s - 1)) % s) return fail("band solve: must start on a full-step level");
+    if (stages < 0) return fail("band solve: negative stage count");
+    const int wout = 128 - 2 * H;
+    ps_band_grid g;
+    g.nx = p->nx;
+    g.ny = p->ny;
+    g.T = p->T;
+    g.rows = p->rows;
+    g.bands = (p->ny + wout - 1) / wout;
+    g.segs = (p->nx + p->rows - 1) / p->rows;
+    const long long waves = static_cast<long long>(g.bands) * g.segs;
+    if (waves > (1LL << 30)) return fail("band solve: grid too large");
+    const unsigned groups = static_cast<unsigned>((waves + 3) / 4);
+    void *state = p->state;
+    for (int done = 0; done < stages;) {
+        g.hh = stages - done < H ? stages - done : H;
+        g.wlo = p->store_all ? 1 : (g.hh - s + 1 > 1 ? g.hh - s + 1 : 1);
+        int gts = A - (s - 1) + 1;
+        void *args[] = {&state, &gts, &A, &g};
+        if (int rc = launch_kind(p->band, groups, 256, args, PS_K_BAND)) return rc;
+        A += g.hh;
+        done += g.hh;
+    }
+    return 0;
+}
+
+int ps_band_run(const ps_band_plan *p, int A, int stages)
+{
+    PS_NEED_INIT();
+    return band_run(p, A, stages);
+}
+
+int ps_band_run_timed(const ps_band_plan *p, int A, int stages, ps_run_stats *stats)
+{
+    PS_NEED_INIT();
+    timer_begin(stats);
+    int rc = band_run(p, A, stages);
     int rc2 = timer_end();
     return rc ? rc : rc2;
 }

@@ -214,6 +214,48 @@ class SweptRun(_Run):
         return done
 
 
+class BandRun(_Run):
+    """Swept solve of a single-variable radius-1 plugin with a point update
+    (heat) through the band kernel: every cell advances ``levels`` levels
+    per launch with the intermediate levels in registers, the blocks' sloped
+    edges recomputed from a halo instead of exchanged through bridge phases
+    (band.inc).  The same stages as ``SweptRun`` for the same schedule, on
+    a ring of levels + s levels; results bitwise equal."""
+
+    def __init__(self, cu_path, nv, ops, its, nx, ny, levels=None, rows=None, real="double", defines=()):
+        self.spec = kernels.band_spec(cu_path, nv, ops, its, real=real, levels=levels, defines=defines)
+        super().__init__(nv, its, nx, ny, self.spec.levels + its, real)
+        mod = kernels.load(self.spec)
+        self.modules = [mod]
+        self.rows = int(rows or kernels.BAND_ROWS)
+        if self.rows < 1:
+            raise ValueError("band rows must be positive")
+        self.plan = hip.BandPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, levels=self.spec.levels,
+                                 rows=self.rows, store_all=0, band=mod.get_function("Band"))
+
+    def run(self, times, on_level=None, stats=None):
+        """The ``times.stages + times.tail`` stages of a swept schedule
+        (schedule.SweptTimes) from level s-1; ``on_level(A)`` for every
+        full-step level as soon as it is complete; ``stats`` (hip.RunStats)
+        accumulates per-launch event timings.  Returns the final level."""
+        s = self.its
+        A, total = s - 1, times.stages + times.tail
+        if on_level is None:
+            self.plan.store_all = 0
+            hip.band_run(self.plan, A, total, stats)
+            return A + total
+        self.plan.store_all = 1
+        done = 0
+        while done < total:
+            hh = min(self.spec.levels, total - done)
+            hip.band_run(self.plan, A, hh, stats)
+            for lv in schedule.full_step_levels(s, A, A + hh):
+                on_level(lv)
+            A += hh
+            done += hh
+        return A
+
+
 def launch_info(run):
     """Registers / LDS of the loaded kernels (for reports)."""
     out = {}

@@ -95,6 +95,31 @@ __device__ void step(double *shared_state, int idx, int globalTimeStep)
 }
 
 /*
+ * Point update used by the band kernel (PS_HAS_POINT): one cell from its
+ * level r-1 value c, x neighbours xm/xp, y neighbours ym/yp and its level
+ * r-2 value prev, all widened to double; the arithmetic and its order are
+ * heat_step's, so results are bitwise the same.  The scheme is a template
+ * argument V (ps_point_variant() picks it once per launch: 1 forward Euler,
+ * 0 RK2), so the kernel's levels carry no branch on it.
+ */
+#define PS_HAS_POINT 1
+#define PS_POINT_VARIANTS 2
+
+__device__ __forceinline__ int ps_point_variant() { return SCHEME ? 1 : 0; }
+
+template <class R, int V>
+__device__ __forceinline__ R ps_point(double c, double xm, double xp, double ym, double yp, double prev,
+                                      int globalTimeStep)
+{
+    const double lx = COURANT * (xp - 2 * c + xm);
+    const double ly = COURANT * (yp - 2 * c + ym);
+    const double d = lx + ly;
+    if (V == 1) return heat_store<R>(d + c, globalTimeStep);
+    if (globalTimeStep % 2 == 0) return heat_store<R>(1.0 * d + prev, globalTimeStep);
+    return heat_store<R>(0.5 * d + c, globalTimeStep);
+}
+
+/*
  * Level update used by the pysweep_amd kernels (PS_HAS_LEVEL): the cells of
  * [x0,x1) x [y0,y1) of one level, lane t taking cells t, t + PS_NT, ...
  * (row-major, y fastest).  cur is level r-1 with the tile strides, prev

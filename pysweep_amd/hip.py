@@ -45,6 +45,11 @@ class StandardPlan(ctypes.Structure):
                 ("bx", c_int), ("by", c_int), ("threads", c_int), ("standard", c_void_p)]
 
 
+class BandPlan(ctypes.Structure):
+    _fields_ = [("state", c_void_p), ("nx", c_int), ("ny", c_int), ("its", c_int), ("T", c_int),
+                ("levels", c_int), ("rows", c_int), ("store_all", c_int), ("band", c_void_p)]
+
+
 class Inst(ctypes.Structure):
     """ps_inst: one problem of an instance-table launch."""
     _fields_ = [("ring", c_void_p), ("consts", c_void_p), ("nx", c_int), ("ny", c_int), ("T", c_int),
@@ -65,7 +70,7 @@ class RunStats(ctypes.Structure):
     """Per-kernel launch time totals (index: KERNEL_KINDS) of the timed
     launches: every launch by default, or one in ``every`` launches of the
     kernels named in ``kinds``."""
-    _fields_ = [("ms", ctypes.c_double * 6), ("launches", ctypes.c_longlong * 6), ("kinds", c_int),
+    _fields_ = [("ms", ctypes.c_double * 7), ("launches", ctypes.c_longlong * 7), ("kinds", c_int),
                 ("every", c_int)]
 
     @classmethod
@@ -81,7 +86,7 @@ class RunStats(ctypes.Structure):
                 for i, k in enumerate(KERNEL_KINDS)}
 
 
-KERNEL_KINDS = ("UpPyramid", "YBridge", "XBridge", "Octahedron", "DownPyramid", "Standard")
+KERNEL_KINDS = ("UpPyramid", "YBridge", "XBridge", "Octahedron", "DownPyramid", "Standard", "Band")
 
 
 def _sig(name, *argtypes, restype=c_int):
@@ -138,6 +143,8 @@ _swept_run_timed = _sig("ps_swept_run_timed", ctypes.POINTER(SweptPlan), c_int, 
                         ctypes.POINTER(RunStats))
 _standard_run_timed = _sig("ps_standard_run_timed", ctypes.POINTER(StandardPlan), c_int, c_int,
                            ctypes.POINTER(RunStats))
+_band_run = _sig("ps_band_run", ctypes.POINTER(BandPlan), c_int, c_int)
+_band_run_timed = _sig("ps_band_run_timed", ctypes.POINTER(BandPlan), c_int, c_int, ctypes.POINTER(RunStats))
 _copy_f64 = _sig("ps_copy_f64", c_void_p, c_void_p, c_size_t)
 _level_sums = _sig("ps_level_sums", c_void_p, c_int, c_int, c_size_t, c_void_p)
 _level_sums_noalloc = _sig("ps_level_sums_noalloc", c_void_p, c_int, c_int, c_size_t, c_void_p)
@@ -158,7 +165,8 @@ EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shut
            "ps_memcpy_htod_async", "ps_memcpy_dtoh_async", "ps_memcpy_dtod_async", "ps_launch", "ps_synchronize",
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_query",
            "ps_event_destroy",
-           "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed", "ps_copy_f64",
+           "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed",
+           "ps_band_run", "ps_band_run_timed", "ps_copy_f64",
            "ps_level_sums", "ps_level_sums_noalloc",
            "ps_multi_index", "ps_multi_run", "ps_multi_run_timed", "ps_graph_begin", "ps_graph_end", "ps_graph_launch",
            "ps_graph_destroy")
@@ -457,6 +465,13 @@ def swept_run_timed(plan, G, c_begin, c_end, stats):
 
 def standard_run_timed(plan, g_begin, g_end, stats):
     check(_standard_run_timed(ctypes.byref(plan), g_begin, g_end, ctypes.byref(stats)), "ps_standard_run_timed")
+
+
+def band_run(plan, A, stages, stats=None):
+    if stats is None:
+        check(_band_run(ctypes.byref(plan), A, stages), "ps_band_run")
+    else:
+        check(_band_run_timed(ctypes.byref(plan), A, stages, ctypes.byref(stats)), "ps_band_run_timed")
 
 
 def multi_index(plan):

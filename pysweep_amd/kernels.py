@@ -62,6 +62,7 @@ class ModuleSpec:
     pfd: int = 1          # swept: levels of face prefetch in flight (level update)
     unroll: int = 14      # swept: phases of at most this many levels run their level loop unrolled
     defines: tuple = ()   # macros defined before the plugin source (e.g. EU_EXACT_DIVISION)
+    levels: int = 0       # band: most levels per launch
 
     @property
     def real_bytes(self):
@@ -74,6 +75,9 @@ class ModuleSpec:
             return compiler.swept_source(user, name, self.nv, self.ops, self.its, self.b, self.slots, self.threads,
                                          real=self.real, multi=self.multi, generic=self.generic, pad=self.pad,
                                          pfd=self.pfd, defines=self.defines, unroll=self.unroll)
+        if self.kind == "band":
+            return compiler.band_source(user, name, self.nv, self.ops, self.its, self.levels, real=self.real,
+                                        defines=self.defines)
         if self.kind == "standard":
             return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
                                             mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic,
@@ -84,6 +88,8 @@ class ModuleSpec:
         return compiler.constant_layout(read_plugin_source(self.user_path))
 
     def lds_bytes(self):
+        if self.kind == "band":
+            return 0
         if self.kind == "swept":
             tile = self.slots * self.nv * self.b * (self.b + self.pad) * self.real_bytes
             return tile + level_scratch_bytes(self.user_path, self.generic, self.b, self.b + self.pad,
@@ -95,7 +101,8 @@ class ModuleSpec:
 
     def describe(self):
         g = (f"b={self.b} slots={self.slots}" + (f" pad={self.pad}" if self.pad else "")
-             if self.kind == "swept" else f"tile={self.bx}x{self.by}")
+             if self.kind == "swept" else f"levels={self.levels}" if self.kind == "band"
+             else f"tile={self.bx}x{self.by}")
         return (f"{self.kind}{'-multi' if self.multi else ''}{'-generic' if self.generic else ''} "
                 f"{os.path.basename(self.user_path)} {self.real} "
                 f"nv={self.nv} k={self.ops} s={self.its} {g} nt={self.threads}"
@@ -105,6 +112,46 @@ class ModuleSpec:
 def has_level_update(user_path):
     """True when the plugin provides the cooperative level update."""
     return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_LEVEL\b", read_plugin_source(user_path), re.M) is not None
+
+
+def has_point_update(user_path):
+    """True when the plugin provides the point update of the band kernel
+    (PS_HAS_POINT: a single-variable radius-1 stencil whose cell update
+    takes its neighbours' values)."""
+    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_POINT\b", read_plugin_source(user_path), re.M) is not None
+
+
+#: band kernel default: output rows per wavefront
+BAND_ROWS = 128
+
+
+def band_levels(its, levels=None, real="double"):
+    """Levels per band launch: a multiple of ITS (so that every launch but
+    the last starts on a full step), even, at most 16.  Measured on MI355X
+    (heat 8192^2, forward Euler, 1000 steps; GCUPS with 64 / 128 / 192 / 256
+    output rows per wavefront): FP64 4 levels 1064 / 1000 / 1091 / 914, 8
+    levels 1757 / 1820 / 1720 / 1652, 10 levels 1845 / 1924 / 1825 / 1813,
+    12 levels 1833 / 1970 / 1868 / 1856, 16 levels 1302 / 1356 / 1427 /
+    1465 (the halo columns recomputed grow with the depth, and past 12 the
+    register history leaves fewer wavefronts per SIMD); FP32 state, whose
+    stochastic rounding costs two integer multiplies per cell, 8 levels
+    906 / 935 / - / 766, 12 levels 793 / 851 / - / 712, 16 levels 511 /
+    519 / - / 560."""
+    if levels is None:
+        levels = 12 if real == "double" else 8
+    h = int(levels)
+    if h < 2 or h > 16 or h % 2 or h % its:
+        raise ValueError(f"band levels {h} must be even, a multiple of intermediate_steps={its}, and 2..16")
+    return h
+
+
+def band_spec(user_path, nv, ops, its, real="double", levels=None, defines=()):
+    if nv != 1 or ops != 1 or its > 2:
+        raise ValueError("the band kernel serves single-variable radius-1 plugins with at most 2 stages")
+    if not has_point_update(user_path):
+        raise ValueError(f"{os.path.basename(user_path)} has no point update (PS_HAS_POINT)")
+    return ModuleSpec("band", os.path.abspath(user_path), nv, ops, its, threads=256, real=real,
+                      levels=band_levels(its, levels, real), defines=tuple(defines))
 
 
 def swept_faces(b, ops):
@@ -385,6 +432,9 @@ def shipped_module_specs():
     # left the fast division's range)
     specs.append(swept_spec(euler, 4, 2, 2, 16, defines=("EU_EXACT_DIVISION",)))
     specs.append(standard_spec(euler, 4, 2, 2, 64, 64, defines=("EU_EXACT_DIVISION",)))
+    for s_ in (1, 2):
+        specs.append(band_spec(heat, 1, 1, s_))
+    specs.append(band_spec(eq_path("heat_fp32.cu"), 1, 1, 1, real="float"))
     for name, nv, k, s in (("euler_fp32.cu", 4, 2, 2), ("heat_fp32.cu", 1, 1, 1)):
         specs.append(standard_spec(eq_path(name), nv, k, s, 1024, 1024, real="float"))
         for b in ((16, 32) if nv > 1 else (16, 32, 64)):

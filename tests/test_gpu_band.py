@@ -1,0 +1,177 @@
+"""GPU parity of the band kernel (the swept solve of single-variable radius-1
+plugins with a point update, heat here) through the C ABI: against the
+oracle on ragged, odd and tiny grids (periodic wrap of rows and columns
+several times over), for every launch depth and row segment, FP64 and FP32
+state, forward Euler and RK2, with and without the full history.  Bitwise
+throughout: the point update evaluates the heat step's arithmetic in its
+order."""
+import numpy as np
+import pytest
+
+from gpu_helpers import gpu_solve, heat_case, o_heat, oracle_stages
+
+pytestmark = pytest.mark.gpu
+
+
+def _heat_grid(nx, ny, fe, steps, seed=0):
+    rng = np.random.default_rng(seed)
+    dx = 1.0 / nx
+    dt = 0.1 * dx * dx
+    x = np.linspace(0, 1, nx, endpoint=False)
+    y = np.linspace(0, 1, ny, endpoint=False)
+    ic = (np.outer(np.sin(2 * np.pi * x), np.cos(4 * np.pi * y)) + 1e-3 * rng.random((nx, ny)))[None]
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, fe]
+    c = o_heat.courant_py(1.0, dt, dx)
+    step = lambda q, qb, gg: o_heat.step_py(q[0], None if qb is None else qb[0], gg, c, fe)[None]
+    return ic, g, step
+
+
+def _band(ic, g, its, steps, levels=None, rows=None, real="double", cu="heat.cu", on_level=None):
+    from pysweep_amd import device as dev, hip, plugins, schedule
+    from pysweep_amd.equations import path as eq_path
+    nv, nx, ny = ic.shape
+    cpu = plugins.load_cpu_module(eq_path("heat.py"))
+    cpu.set_globals(*g)
+    run = dev.BandRun(eq_path(cu), 1, 1, its, nx, ny, levels=levels, rows=rows, real=real)
+    run.set_globals(cpu, g)
+    run.upload_ic(ic)
+    times = schedule.standard_times(steps, 8, 1, its)
+    last = run.run(times, on_level=on_level)
+    hip.synchronize()
+    out = run.level(last)
+    assert last == its - 1 + its * steps
+    return run, out
+
+
+@pytest.mark.parametrize("fe", [True, False], ids=["fe", "rk2"])
+@pytest.mark.parametrize("nx,ny", [(64, 96), (130, 66), (37, 45), (3, 200), (200, 3), (5, 7), (96, 257)])
+def test_band_matches_oracle_on_ragged_and_tiny_grids(device, nx, ny, fe):
+    """Odd ny (no paired loads), ny below one band (columns wrap several
+    times), nx below the halo (rows wrap several times), bands and row
+    segments that do not divide the grid; 13 steps, so the last launch is
+    shorter than the others."""
+    its = 1 if fe else 2
+    steps = 13
+    ic, g, step = _heat_grid(nx, ny, fe, steps, seed=nx + ny)
+    run, got = _band(ic, g, its, steps, rows=48)
+    assert np.array_equal(got, oracle_stages(step, ic, its, its * steps))
+    run.close()
+
+
+@pytest.mark.parametrize("levels", [2, 4, 6, 8, 12, 16])
+def test_band_launch_depths_give_the_same_bits(device, levels):
+    """Every depth (levels per launch) and several row segments equal the
+    standard kernel bitwise (heat 384 x 320, forward Euler, 50 steps)."""
+    from pysweep_amd import device as dev, plugins
+    from pysweep_amd.equations import path as eq_path
+    nx, ny, steps = 384, 320, 50
+    ic, g, _ = _heat_grid(nx, ny, True, steps, seed=3)
+    cpu = plugins.load_cpu_module(eq_path("heat.py"))
+    cpu.set_globals(*g)
+    st = dev.StandardRun(eq_path("heat.cu"), 1, 1, 1, nx, ny)
+    st.set_globals(cpu, g)
+    st.upload_ic(ic)
+    st.run(1, steps + 1)
+    want = st.level(steps)
+    st.close()
+    for rows in (1, 17, 128, 1000):
+        run, got = _band(ic, g, 1, steps, levels=levels, rows=rows)
+        assert np.array_equal(got, want), rows
+        run.close()
+
+
+def test_band_rk2_levels_are_multiples_of_its(device):
+    """RK2 launches start on full steps: depths that are not a multiple of
+    the two stages are refused before anything runs."""
+    from pysweep_amd import kernels
+    from pysweep_amd.equations import path as eq_path
+    with pytest.raises(ValueError):
+        kernels.band_spec(eq_path("heat.cu"), 1, 1, 2, levels=6 + 1)
+    kernels.band_spec(eq_path("heat.cu"), 1, 1, 2, levels=6)
+
+
+def test_band_run_rejects_a_start_off_a_full_step(device):
+    """ps_band_run checks its arguments on the host and launches nothing
+    when they are invalid."""
+    from pysweep_amd import hip
+    ic, g, _ = _heat_grid(32, 32, False, 4)
+    run, _ = _band(ic, g, 2, 4)
+    with pytest.raises(hip.HipError, match="full-step"):
+        hip.band_run(run.plan, 2, 4)
+    bad = hip.BandPlan(state=run.plan.state, nx=32, ny=32, its=2, T=run.T, levels=7, rows=8, store_all=0,
+                       band=run.plan.band)
+    with pytest.raises(hip.HipError, match="levels"):
+        hip.band_run(bad, 1, 4)
+    run.close()
+
+
+@pytest.mark.parametrize("fe", [True, False], ids=["fe", "rk2"])
+def test_band_history_equals_oracle_every_step(device, fe):
+    """With on_level (the Solver's history and HDF5 output) every level is
+    written and each full step handed over is the oracle's."""
+    its = 1 if fe else 2
+    steps = 21
+    ic, g, step = _heat_grid(96, 64, fe, steps, seed=7)
+    seen = {}
+    holder = {}
+
+    def on_level(A):
+        seen[A] = holder["run"].level(A)
+
+    from pysweep_amd import device as dev, hip, plugins, schedule
+    from pysweep_amd.equations import path as eq_path
+    cpu = plugins.load_cpu_module(eq_path("heat.py"))
+    cpu.set_globals(*g)
+    run = dev.BandRun(eq_path("heat.cu"), 1, 1, its, 96, 64, rows=40)
+    holder["run"] = run
+    run.set_globals(cpu, g)
+    run.upload_ic(ic)
+    run.run(schedule.standard_times(steps, 8, 1, its), on_level=on_level)
+    hip.synchronize()
+    assert sorted(seen) == [its - 1 + its * m for m in range(1, steps + 1)]
+    for m in (1, 2, 8, 9, 17, steps):
+        assert np.array_equal(seen[its - 1 + its * m], oracle_stages(step, ic, its, its * m)), m
+    run.close()
+
+
+def test_solver_history_through_the_band_kernel(device):
+    """The Solver's history (the reference's output dataset) with the
+    reference's step quantization, through the band kernel."""
+    n, steps = 64, 30
+    ic, g, step = heat_case(n, False, steps, seed=4)
+    sol = gpu_solve("heat", ic, g, 1, 2, 8, swept=True, exact=False, history=True)
+    from pysweep_amd import device as dev
+    assert isinstance(sol._run, dev.BandRun)
+    for m in range(sol.data.shape[0]):
+        assert np.array_equal(sol.data[m], oracle_stages(step, ic, 2, 2 * m)), m
+
+
+def test_band_fp32_state_equals_the_other_kernels(device):
+    """FP32 state (stochastic storage rounding keyed on the value and the
+    stage): the band kernel gives the bits of the FP32 standard kernel and
+    of the FP32 phase kernels, 1024^2, 100 steps."""
+    from pysweep_amd import device as dev, hip, plugins, schedule, workloads
+    from pysweep_amd.equations import path as eq_path
+    n, steps = 1024, 100
+    dx = 1.0 / n
+    dt = 0.1 * dx * dx
+    g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
+    ic = workloads.heat_mode(n, n)
+    cpu = plugins.load_cpu_module(eq_path("heat.py"))
+    cpu.set_globals(*g)
+    st = dev.StandardRun(eq_path("heat_fp32.cu"), 1, 1, 1, n, n, real="float")
+    st.set_globals(cpu, g)
+    st.upload_ic(ic)
+    st.run(1, steps + 1)
+    want = st.level(steps)
+    st.close()
+    run, got = _band(ic, g, 1, steps, real="float", cu="heat_fp32.cu")
+    assert np.array_equal(got, want)
+    run.close()
+    sw = dev.SweptRun(eq_path("heat_fp32.cu"), 1, 1, 1, n, n, 32, real="float")
+    sw.set_globals(cpu, g)
+    sw.upload_ic(ic)
+    last = sw.run(schedule.swept_times(steps, 32, 1, 1, exact=True))
+    hip.synchronize()
+    assert np.array_equal(sw.level(last), want)
+    sw.close()

@@ -19,12 +19,18 @@ TOL = 1e-12
 
 
 @pytest.mark.parametrize("fe", [True, False], ids=["fe", "rk2"])
-@pytest.mark.parametrize("swept,b", [(False, 8), (True, 8), (True, 16), (True, 32)])
-def test_heat_matches_oracle(device, fe, swept, b):
+@pytest.mark.parametrize("swept,b,phases", [(False, 8, False), (True, 8, False), (True, 32, False), (True, 8, True),
+                                            (True, 16, True), (True, 32, True)])
+def test_heat_matches_oracle(device, fe, swept, b, phases):
+    """Heat swept solves run the band kernel by default and the five phase
+    kernels with ``phases``; both, and the standard kernel, give the
+    oracle's bits."""
     n, steps = 64, 40
     s = 1 if fe else 2
     ic, g, step = heat_case(n, fe, steps, seed=1)
-    sol = gpu_solve("heat", ic, g, 1, s, b, swept=swept, exact=True)
+    sol = gpu_solve("heat", ic, g, 1, s, b, swept=swept, exact=True, tuning={"phases": phases})
+    from pysweep_amd import device as dev
+    assert isinstance(sol._run, dev.BandRun) == (swept and not phases)
     want = oracle_stages(step, ic, s, s * steps)
     assert normwise(sol.final_state, want) <= TOL
     assert np.array_equal(sol.final_state, want)
@@ -77,16 +83,17 @@ def test_euler_one_stage_parity_with_reference_pow_along_trajectory(device):
     run.close()
 
 
-def _swept_vs_standard(name, ic, g, k, s, b, steps):
-    a = gpu_solve(name, ic, g, k, s, b, swept=True, exact=True)
+def _swept_vs_standard(name, ic, g, k, s, b, steps, tuning=None):
+    a = gpu_solve(name, ic, g, k, s, b, swept=True, exact=True, tuning=tuning)
     c = gpu_solve(name, ic, g, k, s, b, swept=False, exact=True)
     return a.final_state, c.final_state
 
 
+@pytest.mark.parametrize("phases", [True, False], ids=["phases", "band"])
 @pytest.mark.parametrize("b", [8, 16, 32])
-def test_heat_swept_equals_standard_bitwise(device, b):
+def test_heat_swept_equals_standard_bitwise(device, b, phases):
     ic, g, _ = heat_case(128, True, 100, seed=2)
-    a, c = _swept_vs_standard("heat", ic, g, 1, 1, b, 100)
+    a, c = _swept_vs_standard("heat", ic, g, 1, 1, b, 100, tuning={"phases": phases})
     assert np.array_equal(a, c)
 
 
@@ -187,13 +194,14 @@ def test_known_answer_half(device, swept):
 
 # ---------------------------------------------------------------- configurations
 
-def test_config_heat_384_swept_cpu_case(device):
+@pytest.mark.parametrize("phases", [True, False], ids=["phases", "band"])
+def test_config_heat_384_swept_cpu_case(device, phases):
     """Heat 384^2, b=8, forward Euler, 100 steps with the reference's
     quantization (99 stages): oracle bitwise, exact discrete Fourier solution
     and the analytic decay of the sin(2 pi x) sin(2 pi y) mode."""
     n, steps = 384, 100
     ic, g, step = heat_case(n, True, steps, seed=0)
-    sol = gpu_solve("heat", ic, g, 1, 1, 8, swept=True, exact=False)
+    sol = gpu_solve("heat", ic, g, 1, 1, 8, swept=True, exact=False, tuning={"phases": phases})
     assert sol.times.stages == 99 and sol.times.G == 32
     want = oracle_stages(step, ic, 1, 99)
     assert np.array_equal(sol.final_state, want)
@@ -260,8 +268,8 @@ def test_standard_ragged_tiles_heat(device, fe):
     assert np.array_equal(sol.final_state, want)
 
 
-@pytest.mark.parametrize("b", [32, 64])
-def test_large_heat_swept_equals_standard_and_decays(device, b):
+@pytest.mark.parametrize("b,phases", [(32, True), (64, True), (64, False)])
+def test_large_heat_swept_equals_standard_and_decays(device, b, phases):
     """Heat 8192^2 (the large-instance size), 200 forward-Euler steps:
     swept == standard bitwise; the sin(2 pi x) sin(2 pi y) mode decays as
     exp(-8 pi^2 t) to O(dx^2)."""
@@ -271,7 +279,7 @@ def test_large_heat_swept_equals_standard_and_decays(device, b):
     dt = 0.1 * dx * dx
     ic = workloads.heat_mode(n, n)
     g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
-    sw = gpu_solve("heat", ic, g, 1, 1, b, swept=True, exact=True)
+    sw = gpu_solve("heat", ic, g, 1, 1, b, swept=True, exact=True, tuning={"phases": phases})
     st = gpu_solve("heat", ic, g, 1, 1, b, swept=False, exact=True)
     assert np.array_equal(sw.final_state, st.final_state)
     ratio = sw.final_state[0, n // 4, n // 4] / ic[0, n // 4, n // 4]
@@ -314,9 +322,11 @@ def test_reference_sweep_heat_rk2_384(device, b, swept):
     g = [0.0, steps * dt, dt, dx, dx, 1.0, False]
     c = o_heat.courant_py(1.0, dt, dx)
     step = lambda q, qb, gg: o_heat.step_py(q[0], qb[0], gg, c, False)[None]
-    sol = gpu_solve("heat", ic, g, 1, 2, b, swept=swept, exact=True)
-    assert np.array_equal(sol.final_state, oracle_stages(step, ic, 2, 2 * steps))
-    assert np.allclose(sol.final_state[0], ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt))
+    want = oracle_stages(step, ic, 2, 2 * steps)
+    for phases in ((True, False) if swept else (False,)):
+        sol = gpu_solve("heat", ic, g, 1, 2, b, swept=swept, exact=True, tuning={"phases": phases})
+        assert np.array_equal(sol.final_state, want)
+        assert np.allclose(sol.final_state[0], ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt))
 
 
 @pytest.mark.parametrize("b", [12, 24])
@@ -340,9 +350,10 @@ def test_reference_heat_forward_euler_120_b24(device):
     x = o_heat.grid(n)
     ic = np.outer(np.sin(2 * np.pi * x), np.sin(2 * np.pi * x))[None]
     g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
-    sol = gpu_solve("heat", ic, g, 1, 1, 24, swept=True, exact=True)
     exact = ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt)
-    assert np.allclose(sol.final_state[0], exact)
+    for phases in (True, False):
+        sol = gpu_solve("heat", ic, g, 1, 1, 24, swept=True, exact=True, tuning={"phases": phases})
+        assert np.allclose(sol.final_state[0], exact)
 
 
 def test_reference_euler_vortex_120_b24(device):
@@ -488,7 +499,7 @@ def test_face_prefetch_depth_heat(device, pfd):
     heat 256^2, b=64 (31-level phases, the rolled level loop unrolled by
     the depth, including depths that do not divide the level count)."""
     ic, g, step = heat_case(256, 1, 100, seed=2)
-    s = gpu_solve("heat", ic, g, 1, 1, 64, tuning={"pfd": pfd})
+    s = gpu_solve("heat", ic, g, 1, 1, 64, tuning={"pfd": pfd, "phases": True})
     assert np.array_equal(s.final_state, oracle_stages(step, ic, 1, 100))
 
 
@@ -551,10 +562,10 @@ def test_nan_poisoned_ring_swept_equals_oracle(device, name, b, k, s, fe):
         ic, g, step = euler_case(128, 12)
         stages = 24
     want = oracle_stages(step, ic, s, stages)
-    for swept in (True, False):
-        r = gpu_solve(name, ic, g, k, s, b, swept=swept, tuning={"poison": True})
+    for swept, phases in ((True, True), (True, False), (False, False)):
+        r = gpu_solve(name, ic, g, k, s, b, swept=swept, tuning={"poison": True, "phases": phases})
         assert np.isfinite(r.final_state).all()
-        assert np.array_equal(r.final_state, want), swept
+        assert np.array_equal(r.final_state, want), (swept, phases)
 
 
 def test_nan_poisoned_engine_rings(device):

@@ -50,6 +50,18 @@ def test_shipped_plugins_compile_for_gfx950(name, nv, k, s):
             assert size == 0, f"{spec.describe()}: {kernel} uses {size} B of scratch memory"
 
 
+@pytest.mark.parametrize("name,s,real", [("heat.cu", 1, "double"), ("heat.cu", 2, "double"),
+                                         ("heat_fp32.cu", 1, "float")])
+def test_band_kernels_compile_without_scratch(name, s, real):
+    """The band kernel keeps every level's history in registers: it must
+    compile to a code object with no scratch memory (a spill would put the
+    history in per-lane memory)."""
+    code = kernels.build_module_code(kernels.band_spec(eq_path(name), 1, 1, s, real=real))
+    assert code[:4] == b"\x7fELF"
+    sizes = _private_segment_sizes(code)
+    assert not os.path.exists(READELF) or sizes.get("Band") == 0, sizes
+
+
 READELF = "/opt/rocm/llvm/bin/llvm-readelf"
 
 
