@@ -6,8 +6,12 @@ dt = 0.1*dx, blocksize 16, operating_points 2, intermediate_steps 2,
 500 full steps (1000 RK2 stages), float64, state resident in HBM.
 
 One bench step = one complete solve of that workload.  ``value`` is the
-swept solve's GCUPS = nx*ny*500 / (seconds per solve) / 1e9; the standard
-kernel's GCUPS on the same inputs is reported beside it.
+swept solve's GCUPS = nx*ny*500 / (seconds per solve) / 1e9, the swept solve
+being the product's default for the Euler plugin: the band kernel (one RK2
+step per launch, every level in registers, the blocks' sloped edges
+recomputed instead of bridged).  The standard kernel's GCUPS and the five
+phase kernels' (the swept rule's pyramids, bridges and octahedra, b=16) on
+the same inputs are reported beside it.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
 
@@ -36,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_euler1024_b16.json")
 #: the kernel build the committed counter file was measured on (bumped when
 #: the swept or standard kernels change)
-KERNEL_BUILD = 3
+KERNEL_BUILD = 4
 
 
 def parse():
@@ -49,7 +53,9 @@ def parse():
     ap.add_argument("--nsteps", type=int, default=500)
     ap.add_argument("--cpu-stages", type=int, default=12)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["swept", "standard", "both"], default="both")
+    ap.add_argument("--only", choices=["swept", "standard", "phases", "both", "all"], default="all",
+                    help="legs of the headline: swept (band kernel), standard, phases (five phase kernels); "
+                         "both = swept and standard, all = the three")
     ap.add_argument("--batch", type=int, default=512, help="problems of the batched workload (0: skip)")
     ap.add_argument("--batch-legs", default="tuned,swept,standard,plain",
                     help="runs of the batched workload: engine with swept / standard kernels, plain one at a time")
@@ -71,6 +77,9 @@ def parse():
 
 
 def setup(n, b, nsteps, generic=False, slots=None, threads=None):
+    """The headline's runs: the default swept solve (the band kernel; the
+    phase kernels with --generic, whose per-cell step the band sweep does not
+    take), the five phase kernels, and the standard kernel."""
     import numpy as np
 
     from pysweep_amd import device, hip, plugins, schedule, workloads
@@ -83,14 +92,14 @@ def setup(n, b, nsteps, generic=False, slots=None, threads=None):
     g = [0.0, nsteps * dt, dt, dx, dx, 1.4]
     cpu = plugins.load_cpu_module(eq_path("euler.py"))
     cpu.set_globals(*g)
-    sw = device.SweptRun(eq_path("euler.cu"), 4, 2, 2, n, n, b, generic=generic, slots=slots, threads=threads)
-    sw.set_globals(cpu, g)
-    sw.keep_ic_on_device(ic)
+    ph = device.SweptRun(eq_path("euler.cu"), 4, 2, 2, n, n, b, generic=generic, slots=slots, threads=threads)
+    sw = ph if generic else device.BandRun(eq_path("euler.cu"), 4, 2, 2, n, n)
     st = device.StandardRun(eq_path("euler.cu"), 4, 2, 2, n, n, generic=generic)
-    st.set_globals(cpu, g)
-    st.keep_ic_on_device(ic)
+    for r in {id(x): x for x in (sw, ph, st)}.values():
+        r.set_globals(cpu, g)
+        r.keep_ic_on_device(ic)
     times = schedule.swept_times(nsteps, b, 2, 2, exact=True)
-    return np, hip, ic, g, sw, st, times
+    return np, hip, ic, g, sw, ph, st, times
 
 
 SAMPLE_EVERY = 4  # timed region: HIP events around one launch in 4 of the dominant kernel
@@ -281,7 +290,9 @@ HEAT_FLOPS_PER_CELL_STAGE = 12      # heat.cu:11-24 as written
 
 def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float")):
     """Large single instances, 1000 steps, swept vs standard, FP64 and FP32
-    state: heat 8192^2 (b=32, and b=64) and Euler 2048^2 (b=32, and b=16).  FP32
+    state: heat 8192^2 (phase kernels with b=32 and b=64, and the band
+    kernel) and Euler 2048^2 (phase kernels with b=32 and b=16, and the band
+    kernel).  FP32
     errors are stated against the FP64 result (itself bitwise equal to the
     CPU oracle, which the parity tests check at smaller sizes).  ``names``,
     ``steps`` and ``reals`` narrow the study (the profiling runs use them)."""
@@ -312,10 +323,9 @@ def large_instances(names=("heat", "euler"), steps=1000, reals=("double", "float
             if real not in reals:
                 continue
             runs = [("standard", None)] + [("swept", b) for b in bs]
-            if name == "heat":
-                # the swept solve of the band kernel (levels in registers,
-                # halos recomputed instead of bridged)
-                runs.append(("band", None))
+            # the swept solve of the band kernel (levels in registers,
+            # halos recomputed instead of bridged)
+            runs.append(("band", None))
             for method, b in runs:
                 if method == "band":
                     r = device.BandRun(eq_path(cu), nv, k, s, n, n, real=real)
@@ -593,15 +603,18 @@ def main():
     if a.copy_calibration:
         copy_calibration()
         return
-    np, hip, ic, g, sw, st, times = setup(a.n, a.b, a.nsteps, a.generic, a.slots, a.threads)
-    from pysweep_amd import schedule
+    np, hip, ic, g, sw, ph, st, times = setup(a.n, a.b, a.nsteps, a.generic, a.slots, a.threads)
+    from pysweep_amd import device, schedule
+    band = isinstance(sw, device.BandRun)
     cells = a.n * a.n
     res = {}
-    if a.only in ("swept", "both"):
+    if a.only in ("swept", "both", "all"):
         res["swept"] = time_solves(hip, lambda s: sw.run(times, stats=s), sw.reset, a.steps, a.warmup)
-    if a.only in ("standard", "both"):
+    if a.only in ("standard", "both", "all"):
         stages = 2 * a.nsteps
         res["standard"] = time_solves(hip, lambda s: st.run(1, stages + 1, stats=s), st.reset, a.steps, a.warmup)
+    if a.only in ("phases", "all") and band:
+        res["phases"] = time_solves(hip, lambda s: ph.run(times, stats=s), ph.reset, a.steps, a.warmup)
 
     out = {"metric": METRIC, "unit": "GCUPS", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic"}
@@ -611,15 +624,23 @@ def main():
     out["ms_per_step"] = res[head][0] * 1e3
     out["config"] = {"workload": f"euler-vortex-{a.n}x{a.n}-b{a.b}-{a.nsteps}steps-{head}", "nx": a.n, "ny": a.n,
                      "blocksize": a.b, "operating_points": 2, "intermediate_steps": 2, "steps_per_solve": a.nsteps,
-                     "method": head, "G": times.G, "swept_stages": times.stages, "tail_stages": times.tail,
-                     "lds_slots": sw.spec.slots, "threads": sw.spec.threads,
-                     "step": "per-cell plugin step" if a.generic else "plugin level update",
+                     "method": head, "swept_kernel": "band" if band else "phases",
+                     "G": times.G, "swept_stages": times.stages, "tail_stages": times.tail,
+                     "phases_lds_slots": ph.spec.slots, "phases_threads": ph.spec.threads,
+                     "step": "per-cell plugin step" if a.generic else "plugin level update / band sweep",
                      "standard_tile": [st.spec.bx, st.spec.by]}
+    if band:
+        out["config"]["band_levels"] = sw.spec.levels
+        out["config"]["band_rows"] = sw.rows
     if "standard" in gc:
         out["standard_gcups"] = gc["standard"]
         out["standard_ms_per_solve"] = res["standard"][0] * 1e3
     if "swept" in gc and "standard" in gc:
         out["swept_over_standard"] = gc["swept"] / gc["standard"]
+    if "phases" in gc:
+        out["phases_gcups"] = gc["phases"]
+        if "standard" in gc:
+            out["phases_over_standard"] = gc["phases"] / gc["standard"]
 
     # roofline of the dominant kernel of the headline method (events of the
     # timed region, which time that kernel only)
@@ -629,6 +650,8 @@ def main():
     tiles = (a.n // a.b) ** 2
     if kern == "Standard":
         cs = cells
+    elif kern == "Band":
+        cs = cells * sw.spec.levels
     else:
         cs = tiles * schedule.phase_cell_stages(kern, a.b, 2, P)
     avg_s = stats[kern]["avg_ms"] / 1e3
@@ -658,8 +681,11 @@ def main():
         fin_sw = sw.level(times.stages + times.tail + 1)
         fin_st = st.level(2 * a.nsteps + 1)
         out["swept_equals_standard_bitwise"] = bool(np.array_equal(fin_sw, fin_st))
-    sw.close()
-    st.close()
+        if "phases" in res:
+            out["phases_equals_standard_bitwise"] = bool(np.array_equal(ph.level(times.stages + times.tail + 1),
+                                                                        fin_st))
+    for r in {id(x): x for x in (sw, ph, st)}.values():
+        r.close()
     progress(f"headline done: swept {gc.get('swept')} GCUPS, standard {gc.get('standard')} GCUPS")
     if a.large:
         out["large_instances"] = large_instances(names=tuple(a.large_cases.split(",")), steps=a.large_steps,

@@ -235,13 +235,14 @@ typedef struct ps_run_stats {
 int ps_swept_run_timed(const ps_swept_plan *plan, int G, int c_begin, int c_end, ps_run_stats *stats);
 int ps_standard_run_timed(const ps_standard_plan *plan, int gts_begin, int gts_end, ps_run_stats *stats);
 
-/* Band solve: the swept solve of a single-variable plugin on a radius-1
- * cross stencil that provides a point update (PS_HAS_POINT), e.g. heat.
- * Each launch of the `Band` kernel advances every cell `levels` levels (the
- * last launch fewer) with all intermediate levels in registers, one HBM read
- * and one HBM write per cell: bands of 128 columns, each swept along x by
- * one wavefront, whose sloped edges are recomputed from a halo of `levels`
- * columns instead of being exchanged through bridge phases.  Kernel
+/* Band solve: the swept solve of a plugin that provides a point update
+ * (PS_HAS_POINT: single variable, radius-1 cross stencil, e.g. heat) or a
+ * band sweep of its own (PS_HAS_BAND, e.g. Euler).  Each launch of the
+ * `Band` kernel advances every cell `levels` levels (the last launch fewer)
+ * with all intermediate levels in registers, one HBM read and one HBM write
+ * per cell: bands of columns, each swept along x by one wavefront, whose
+ * sloped edges are recomputed from a halo of k columns per level instead of
+ * being exchanged through bridge phases.  Kernel
  * Band(double *state, int globalTimeStep, int level, ps_band_grid g).
  * Replaces Solver.sweptSolve with functions.FirstPrism/UpPrism/LastPrism
  * (core/solver.py:158-176; core/functions.py:12-88) for such plugins: the
@@ -264,16 +265,18 @@ typedef struct ps_band_plan {
     int T;          /* at least levels + s                                 */
     int levels;     /* levels per launch compiled into the kernel (even,
                        a multiple of s, 2..16)                              */
+    int wout;       /* output columns per band (the kernel's band width less
+                       its halo columns: 128 - 2*levels for a point-update
+                       plugin, 64 - 2*k*levels for a plugin sweep)          */
     int rows;       /* output rows per wavefront (> 0)                     */
     int store_all;  /* 1: write every level; 0: the last s of each launch  */
     ps_function band;
 } ps_band_plan;
 
-/* Run stages from absolute level A (which, with the s-1 before it, must be
- * complete in the ring and be a full-step level) for `stages` stages, in
- * launches of plan->levels stages (the last one fewer); the last s levels
- * reached are complete in the ring afterwards, and every level when
- * store_all is set. */
+/* Run stages from absolute level A (which must be complete in the ring and
+ * be a full-step level) for `stages` stages, in launches of plan->levels
+ * stages (the last one fewer); the last s levels reached are complete in the
+ * ring afterwards, and every level when store_all is set. */
 int ps_band_run(const ps_band_plan *plan, int A, int stages);
 int ps_band_run_timed(const ps_band_plan *plan, int A, int stages, ps_run_stats *stats);
 

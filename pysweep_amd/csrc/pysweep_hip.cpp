@@ -761,7 +761,8 @@ static int band_run(const ps_band_plan *p, int A, int stages)
     if (p->nx < 1 || p->ny < 1 || p->rows < 1) return fail("band solve: empty grid or row segment");
     if (A < s - 1 || (A - (s - 1)) % s) return fail("band solve: must start on a full-step level");
     if (stages < 0) return fail("band solve: negative stage count");
-    const int wout = 128 - 2 * H;
+    const int wout = p->wout;
+    if (wout < 1 || wout > 128) return fail("band solve: output columns per band must be 1..128");
     ps_band_grid g;
     g.nx = p->nx;
     g.ny = p->ny;
@@ -775,7 +776,10 @@ static int band_run(const ps_band_plan *p, int A, int stages)
     void *state = p->state;
     for (int done = 0; done < stages;) {
         g.hh = stages - done < H ? stages - done : H;
-        g.wlo = p->store_all ? 1 : (g.hh - s + 1 > 1 ? g.hh - s + 1 : 1);
+        // the next launch reads only its first level (a full step); the
+        // last launch leaves its last s levels complete
+        const bool last = done + g.hh >= stages;
+        g.wlo = p->store_all ? 1 : !last ? g.hh : (g.hh - s + 1 > 1 ? g.hh - s + 1 : 1);
         int gts = A - (s - 1) + 1;
         void *args[] = {&state, &gts, &A, &g};
         if (int rc = launch_kind(p->band, groups, 256, args, PS_K_BAND)) return rc;

@@ -215,23 +215,25 @@ class SweptRun(_Run):
 
 
 class BandRun(_Run):
-    """Swept solve of a single-variable radius-1 plugin with a point update
-    (heat) through the band kernel: every cell advances ``levels`` levels
-    per launch with the intermediate levels in registers, the blocks' sloped
-    edges recomputed from a halo instead of exchanged through bridge phases
-    (band.inc).  The same stages as ``SweptRun`` for the same schedule, on
-    a ring of levels + s levels; results bitwise equal."""
+    """Swept solve through the band kernel, for a plugin with a point update
+    (heat) or a band sweep of its own (Euler): every cell advances
+    ``levels`` levels per launch with the intermediate levels in registers,
+    the blocks' sloped edges recomputed from a halo instead of exchanged
+    through bridge phases (band.inc).  The same stages as ``SweptRun`` for
+    the same schedule, on a ring of levels + s levels; results bitwise
+    equal."""
 
     def __init__(self, cu_path, nv, ops, its, nx, ny, levels=None, rows=None, real="double", defines=()):
         self.spec = kernels.band_spec(cu_path, nv, ops, its, real=real, levels=levels, defines=defines)
         super().__init__(nv, its, nx, ny, self.spec.levels + its, real)
         mod = kernels.load(self.spec)
         self.modules = [mod]
-        self.rows = int(rows or kernels.BAND_ROWS)
+        self.rows = int(rows or kernels.band_rows(self.spec, nx, ny))
         if self.rows < 1:
             raise ValueError("band rows must be positive")
         self.plan = hip.BandPlan(state=self.ring.ptr, nx=nx, ny=ny, its=its, T=self.T, levels=self.spec.levels,
-                                 rows=self.rows, store_all=0, band=mod.get_function("Band"))
+                                 wout=kernels.band_out_columns(self.spec), rows=self.rows, store_all=0,
+                                 band=mod.get_function("Band"))
 
     def run(self, times, on_level=None, stats=None):
         """The ``times.stages + times.tail`` stages of a swept schedule
@@ -260,7 +262,7 @@ def launch_info(run):
     """Registers / LDS of the loaded kernels (for reports)."""
     out = {}
     for m in run.modules:
-        for name in ("UpPyramid", "Octahedron", "Standard"):
+        for name in ("UpPyramid", "Octahedron", "Standard", "Band"):
             try:
                 out[name] = hip.function_info(m.get_function(name))
             except Exception:

@@ -532,3 +532,275 @@ __device__ __forceinline__ void ps_level(const R *cur, const Prev &prev, R *out,
         }
     }
 }
+
+#ifdef PS_BAND_COMMON
+/*
+ * Band sweep (PS_HAS_BAND): the swept solve's kernel with every level of a
+ * launch in registers.  One wavefront owns a band of 64 columns, one column
+ * per lane, and sweeps it along x; a launch of PS_BAND_H stages reads level
+ * A over the band and writes level A + PS_BAND_H over its middle
+ * 64 - 4 * PS_BAND_H columns (the radius-2 stencil costs two halo columns
+ * per level on each side, recomputed by the neighbouring bands instead of
+ * exchanged).
+ *
+ * Step tau brings in row xs + tau of level A (xs = x0 - 2 * hh) and, for
+ * l = 1..hh in order, computes row xs + tau - 2 * l of level A + l.  Each
+ * level keeps the last five rows it produced (states, in the storage type)
+ * and the last four rows' pressures: the x face between rows r and r + 1
+ * needs the pressures of rows r - 1 .. r + 2 and the states of r and r + 1,
+ * and a final RK2 stage needs its cell two levels back, four rows behind.
+ * The x face below a row is the one above the row before, carried from the
+ * previous step; the y face east of a column is computed by its lane from
+ * the neighbouring lanes' states and pressures (DPP), and the west one is
+ * the lower lane's east face.  Every face, pressure and update is evaluated
+ * by the functions and in the order of ps_level and the per-cell step, so
+ * results are bitwise theirs.  Rows and columns that a level cannot yet
+ * compute correctly (warm-up steps, the outermost lanes) hold garbage that
+ * no valid cell ever reads; only valid cells are range-checked and stored.
+ * Launches start on full steps (the first stage of a launch never reads
+ * the level two back).
+ */
+#define PS_HAS_BAND 1
+#define EU_BAND_WOUT (64 - 4 * PS_BAND_H)
+static_assert(EU_BAND_WOUT >= 8, "PS_BAND_H too deep for a 64-column band");
+
+/* face between cells a (low) and b (high) along x (XDIR) or y: pww, pw, pc
+ * and pe are the pressures of the cell below a, a, b and the cell above b;
+ * flux and dissipation as in eu_face */
+template <bool XDIR>
+__device__ __forceinline__ void eu_face_vals(const eu_q &a, const eu_q &b, double pww, double pw, double pc,
+                                             double pe, eu_q &f, eu_q &d)
+{
+    const eu_q L = eu_limit(a, b, pc - pw, pw - pww);
+    const eu_q Rr = eu_limit(b, a, pc - pw, pe - pc);
+    const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
+    f = eu_flux<XDIR>(L, Rr, rl, rr);
+    d = eu_dissipation<XDIR>(L, Rr, rl, rr);
+}
+
+/* eu_face_diff on values: 0.5*((((0 + fa) + da) - fb) - db) */
+__device__ __forceinline__ eu_q eu_band_diff(const eu_q &fa, const eu_q &da, const eu_q &fb, const eu_q &db)
+{
+    eu_q a;
+    a.r = 0.0; a.m = 0.0; a.n = 0.0; a.e = 0.0;
+    a.r = a.r + fa.r; a.m = a.m + fa.m; a.n = a.n + fa.n; a.e = a.e + fa.e;
+    a.r = a.r + da.r; a.m = a.m + da.m; a.n = a.n + da.n; a.e = a.e + da.e;
+    a.r = a.r - fb.r; a.m = a.m - fb.m; a.n = a.n - fb.n; a.e = a.e - fb.e;
+    a.r = a.r - db.r; a.m = a.m - db.m; a.n = a.n - db.n; a.e = a.e - db.e;
+    a.r = a.r * 0.5; a.m = a.m * 0.5; a.n = a.n * 0.5; a.e = a.e * 0.5;
+    return a;
+}
+
+__device__ __forceinline__ eu_q eu_lower(const eu_q &q)
+{
+    eu_q o;
+    o.r = ps_from_lower(q.r); o.m = ps_from_lower(q.m); o.n = ps_from_lower(q.n); o.e = ps_from_lower(q.e);
+    return o;
+}
+
+__device__ __forceinline__ eu_q eu_upper(const eu_q &q)
+{
+    eu_q o;
+    o.r = ps_from_upper(q.r); o.m = ps_from_upper(q.m); o.n = ps_from_upper(q.n); o.e = ps_from_upper(q.e);
+    return o;
+}
+
+template <class R>
+struct eu_band_row {
+    R v[4];
+    __device__ __forceinline__ eu_q q() const
+    {
+        eu_q o;
+        o.r = (double)v[0]; o.m = (double)v[1]; o.n = (double)v[2]; o.e = (double)v[3];
+        return o;
+    }
+};
+
+/* The last three rows a level produced (slot tau % 3 holds the newest, so
+ * nothing moves between steps), their pressures, and each row's pressure
+ * difference to the row before it: the x face (r | r+1) takes
+ * p(r+1) - p(r), p(r) - p(r-1) and p(r+2) - p(r+1) as its limiter ratios'
+ * numerator and denominators. */
+template <class R>
+struct eu_band_win {
+    eu_band_row<R> q[3];
+    double p[3], dp[3];
+};
+
+template <int PH, class R>
+__device__ __forceinline__ void eu_band_push(eu_band_win<R> &w, const eu_band_row<R> &row, bool check)
+{
+    const eu_q q = row.q();
+    const double p = eu_pressure(q);
+    w.q[PH] = row;
+    w.dp[PH] = p - w.p[(PH + 2) % 3];
+    w.p[PH] = p;
+    if (check) eu_range_check(q.r, q.m, q.n, q.e, p);
+}
+
+/* Row r of level l from level l-1's window at step phase PH (its newest
+ * row, slot PH, is r + 2): the x face (r | r+1), the y faces of row r, the
+ * divergences with the x face below (xf, xd, replaced by the new one) and
+ * the RK2 update from base (the cell itself, or for a final stage the cell
+ * two levels back). */
+template <int PH, bool FINAL, class R>
+__device__ __forceinline__ eu_band_row<R> eu_band_cell(const eu_band_win<R> &w, eu_q &xf, eu_q &xd,
+                                                       const eu_q &base2)
+{
+    constexpr int pa = (PH + 1) % 3, pb = (PH + 2) % 3;  /* rows r, r + 1 */
+    const eu_q a = w.q[pa].q(), b = w.q[pb].q();
+    eu_q f, d;
+    {
+        const eu_q L = eu_limit(a, b, w.dp[pb], w.dp[pa]);
+        const eu_q Rr = eu_limit(b, a, w.dp[pb], w.dp[PH]);
+        const eu_rcp rl = eu_recip(L.r), rr = eu_recip(Rr.r);
+        f = eu_flux<true>(L, Rr, rl, rr);
+        d = eu_dissipation<true>(L, Rr, rl, rr);
+    }
+    const double pr = w.p[pa], pu = ps_from_upper(pr);
+    eu_q fe, de;
+    eu_face_vals<false>(a, eu_upper(a), ps_from_lower(pr), pr, pu, ps_from_upper(pu), fe, de);
+    const eu_q fx = eu_band_diff(xf, xd, f, d);
+    const eu_q fy = eu_band_diff(eu_lower(fe), eu_lower(de), fe, de);
+    xf = f;
+    xd = d;
+    const eu_q bs = FINAL ? base2 : a;
+    eu_band_row<R> o;
+    if (FINAL) {
+        o.v[0] = (R)(bs.r + (fx.r * DTDX + fy.r * DTDY));
+        o.v[1] = (R)(bs.m + (fx.m * DTDX + fy.m * DTDY));
+        o.v[2] = (R)(bs.n + (fx.n * DTDX + fy.n * DTDY));
+        o.v[3] = (R)(bs.e + (fx.e * DTDX + fy.e * DTDY));
+    } else {
+        o.v[0] = (R)(bs.r + 0.5 * (fx.r * DTDX + fy.r * DTDY));
+        o.v[1] = (R)(bs.m + 0.5 * (fx.m * DTDX + fy.m * DTDY));
+        o.v[2] = (R)(bs.n + 0.5 * (fx.n * DTDX + fy.n * DTDY));
+        o.v[3] = (R)(bs.e + 0.5 * (fx.e * DTDX + fy.e * DTDY));
+    }
+    return o;
+}
+
+template <class R>
+struct eu_band_ctx {
+    R *state;
+    const R *in;  /* level A */
+    size_t cells;
+    int s0, T, nx, ny, wlo, hh, x0, rows, xs, col, oc, lane;
+    bool out_col;
+};
+
+template <class R>
+__device__ __forceinline__ void eu_band_load(const eu_band_ctx<R> &c, int row, eu_band_row<R> &o)
+{
+    const R *p = c.in + (size_t)row * c.ny + c.col;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) o.v[v] = p[v * c.cells];
+}
+
+template <class R>
+__device__ __forceinline__ void eu_band_store(const eu_band_ctx<R> &c, int l, int row, const eu_band_row<R> &o)
+{
+    if (l >= c.wlo && c.out_col && row >= c.x0 && row < c.x0 + c.rows) {
+        const int slot = c.s0 + l >= c.T ? c.s0 + l - c.T : c.s0 + l;
+        R *dst = c.state + (size_t)slot * NV * c.cells + (size_t)row * c.ny + c.oc;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) dst[v * c.cells] = o.v[v];
+    }
+}
+
+/* wrap a row index into [0, nx) (rows stay within one period of the grid
+ * around the segment only when nx exceeds the halo, so reduce fully) */
+__device__ __forceinline__ int eu_band_wrap(int r, int nx)
+{
+    r %= nx;
+    return r < 0 ? r + nx : r;
+}
+
+/* One step (phase PH = tau % 3) of a one-step (two-stage) launch. */
+template <int PH, class R>
+__device__ __forceinline__ void eu_band_step(const eu_band_ctx<R> &c, int tau, eu_band_win<R> &w0,
+                                             eu_band_win<R> &w1, eu_q (&xf)[2], eu_q (&xd)[2],
+                                             eu_band_row<R> (&pf)[3], int &lr)
+{
+    /* level A: this step's row (loaded three steps ago) */
+    const eu_band_row<R> in = pf[PH];
+    eu_band_load(c, lr, pf[PH]);
+    lr = lr + 1 == c.nx ? 0 : lr + 1;
+    /* the base of the final stage: level A two rows behind level 1's row,
+     * re-read (this wavefront loaded it four steps ago) */
+    eu_band_row<R> b2;
+    const int r2 = c.xs + tau - 4;
+    if (c.hh >= 2 && tau >= 7) eu_band_load(c, eu_band_wrap(r2, c.nx), b2);
+    eu_band_push<PH>(w0, in, true);
+    /* level 1 (a half stage), row xs + tau - 2 */
+    if (tau >= 3) {
+        const eu_band_row<R> v1 = eu_band_cell<PH, false>(w0, xf[0], xd[0], eu_q());
+        const bool ok = tau >= 4 && c.lane >= 2 && c.lane < 62;
+        eu_band_push<PH>(w1, v1, ok);
+        if (ok) eu_band_store(c, 1, c.xs + tau - 2, v1);
+    }
+    /* level 2 (the final stage), row xs + tau - 4 */
+    if (c.hh >= 2 && tau >= 7) {
+        const eu_band_row<R> v2 = eu_band_cell<PH, true>(w1, xf[1], xd[1], b2.q());
+        eu_band_store(c, 2, r2, v2);
+    }
+}
+
+template <class R>
+__device__ __forceinline__ void ps_band_sweep(R *state, int gts0, int A, const ps_band_grid &g)
+{
+    static_assert(PS_BAND_H == 2, "the Euler band sweep takes one RK2 step (two stages) per launch");
+    constexpr int WOUT = EU_BAND_WOUT;
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= g.bands * g.segs) return;  /* whole wavefronts leave */
+    const int band = wave % g.bands, seg = wave / g.bands;
+    (void)gts0;  /* launches start on full steps: level 1 is a half stage, level 2 a final one */
+    eu_band_ctx<R> c;
+    c.state = state;
+    c.cells = (size_t)g.nx * g.ny;
+    c.s0 = A % g.T;
+    c.T = g.T;
+    c.nx = g.nx;
+    c.ny = g.ny;
+    c.in = state + (size_t)c.s0 * NV * c.cells;
+    c.hh = g.hh;
+    c.wlo = g.wlo;
+    c.x0 = seg * g.rows;
+    c.rows = min(g.rows, g.nx - c.x0);
+    c.xs = c.x0 - 2 * g.hh;
+    c.lane = lane;
+    const int o = lane - 2 * PS_BAND_H;  /* band-local output column */
+    c.oc = band * WOUT + o;
+    c.col = eu_band_wrap(c.oc, g.ny);
+    c.out_col = o >= 0 && o < WOUT && c.oc < g.ny;
+    /* defined values in what the warm-up steps read before any level has
+     * produced it (never used by a valid cell) */
+    eu_band_win<R> w0, w1;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w0.q[k].v[v] = w1.q[k].v[v] = (R)0;
+        w0.p[k] = w1.p[k] = w0.dp[k] = w1.dp[k] = 0.0;
+    }
+    eu_q xf[2], xd[2];
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        xf[l].r = xf[l].m = xf[l].n = xf[l].e = 0.0;
+        xd[l] = xf[l];
+    }
+    eu_band_row<R> pf[3];
+    int lr = eu_band_wrap(c.xs, g.nx);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        eu_band_load(c, lr, pf[k]);
+        lr = lr + 1 == g.nx ? 0 : lr + 1;
+    }
+    const int steps = c.rows + 4 * g.hh;
+    for (int tau = 0; tau < steps; tau += 3) {
+        eu_band_step<0>(c, tau, w0, w1, xf, xd, pf, lr);
+        eu_band_step<1>(c, tau + 1, w0, w1, xf, xd, pf, lr);
+        eu_band_step<2>(c, tau + 2, w0, w1, xf, xd, pf, lr);
+    }
+}
+#endif /* PS_BAND_COMMON */

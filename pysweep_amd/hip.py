@@ -47,7 +47,7 @@ class StandardPlan(ctypes.Structure):
 
 class BandPlan(ctypes.Structure):
     _fields_ = [("state", c_void_p), ("nx", c_int), ("ny", c_int), ("its", c_int), ("T", c_int),
-                ("levels", c_int), ("rows", c_int), ("store_all", c_int), ("band", c_void_p)]
+                ("levels", c_int), ("wout", c_int), ("rows", c_int), ("store_all", c_int), ("band", c_void_p)]
 
 
 class Inst(ctypes.Structure):

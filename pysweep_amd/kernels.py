@@ -121,10 +121,6 @@ def has_point_update(user_path):
     return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_POINT\b", read_plugin_source(user_path), re.M) is not None
 
 
-#: band kernel default: output rows per wavefront
-BAND_ROWS = 128
-
-
 def band_levels(its, levels=None, real="double"):
     """Levels per band launch: a multiple of ITS (so that every launch but
     the last starts on a full step), even, at most 16.  Measured on MI355X
@@ -145,13 +141,80 @@ def band_levels(its, levels=None, real="double"):
     return h
 
 
+def has_band_sweep(user_path):
+    """True when the plugin provides a band sweep of its own (PS_HAS_BAND,
+    the Euler plugin), compiled when the band kernel's declarations are
+    present."""
+    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_BAND\b", read_plugin_source(user_path), re.M) is not None
+
+
+def band_capable(user_path, nv, ops, its):
+    """Whether the band kernel can run this plugin's swept solve."""
+    if its > 2:
+        return False
+    if has_point_update(user_path):
+        return nv == 1 and ops == 1
+    return has_band_sweep(user_path)
+
+
+def band_out_columns(spec):
+    """Output columns per band of a band module: a point-update band is 128
+    columns (two per lane) less one halo column per level on each side; a
+    plugin sweep is 64 columns (one per lane) less k per level."""
+    if has_point_update(spec.user_path):
+        return 128 - 2 * spec.levels
+    return 64 - 2 * spec.ops * spec.levels
+
+
+#: wavefront slots the band kernels' launches are balanced over: MI355X has
+#: 256 CUs of 4 SIMDs, and a band wavefront keeps a SIMD busy by itself
+BAND_SIMDS = 1024
+
+
+def band_rows(spec, nx, ny, simds=BAND_SIMDS):
+    """Output rows per wavefront for a band launch over nx x ny: the row
+    segment minimising the busiest SIMD's time (the larger segment on a
+    tie).  That SIMD runs m = ceil(wavefronts / SIMDs) wavefronts.  A
+    wavefront's steps are its segment plus the x halo of the launch, rounded
+    up to the step loop's unrolling (3); level l computes from the step its
+    first valid row appears on; a few steps' worth of set-up is added.  Two
+    or more wavefronts share their SIMD's issue slots in full; one alone
+    stalls on its own dependences and issues about 0.815 of the time.  The
+    constants are fitted to these MI355X measurements, whose ranking the
+    model reproduces (GCUPS): Euler 1024^2, rows 12 11.2, 16 10.0, 18 8.8,
+    20 12.0, 22 12.3, 24 11.1, 28 10.2; Euler 2048^2, rows 16 15.9, 20 15.6,
+    24 14.2, 26 16.0, 28 16.3, 32 14.2; heat 8192^2 at 12 levels, rows 64
+    1833, 128 1970, 192 1868, 256 1856."""
+    wout = band_out_columns(spec)
+    bands = -(-ny // wout)
+    H, k = spec.levels, spec.ops
+    point = has_point_update(spec.user_path)
+    best = None
+    for rows in range(4, min(nx, 256) + 1):
+        waves = bands * -(-nx // rows)
+        steps = 3 * -(-(rows + 2 * k * H) // 3)
+        if point:
+            work = sum(steps - 2 * l for l in range(1, H + 1)) + 2 * H
+        else:
+            work = sum(steps - (2 * k * l - 1) for l in range(1, H + 1)) + 4
+        m = -(-waves // simds)
+        cost = work / 0.815 if m == 1 else m * work
+        if best is None or cost <= best[0]:
+            best = (cost, rows)
+    return best[1]
+
+
 def band_spec(user_path, nv, ops, its, real="double", levels=None, defines=()):
-    if nv != 1 or ops != 1 or its > 2:
-        raise ValueError("the band kernel serves single-variable radius-1 plugins with at most 2 stages")
-    if not has_point_update(user_path):
-        raise ValueError(f"{os.path.basename(user_path)} has no point update (PS_HAS_POINT)")
-    return ModuleSpec("band", os.path.abspath(user_path), nv, ops, its, threads=256, real=real,
+    if not band_capable(user_path, nv, ops, its):
+        raise ValueError(f"{os.path.basename(user_path)} (nv={nv}, k={ops}, s={its}) has no band form: a point "
+                         "update (PS_HAS_POINT, one variable, k=1) or a band sweep (PS_HAS_BAND), s <= 2")
+    if levels is None and has_band_sweep(user_path) and not has_point_update(user_path):
+        levels = its  # one step per launch: the registers hold a few rows of every level
+    spec = ModuleSpec("band", os.path.abspath(user_path), nv, ops, its, threads=256, real=real,
                       levels=band_levels(its, levels, real), defines=tuple(defines))
+    if band_out_columns(spec) < 8:
+        raise ValueError(f"band of {spec.levels} levels leaves fewer than 8 output columns")
+    return spec
 
 
 def swept_faces(b, ops):
@@ -435,6 +498,8 @@ def shipped_module_specs():
     for s_ in (1, 2):
         specs.append(band_spec(heat, 1, 1, s_))
     specs.append(band_spec(eq_path("heat_fp32.cu"), 1, 1, 1, real="float"))
+    specs.append(band_spec(euler, 4, 2, 2))
+    specs.append(band_spec(eq_path("euler_fp32.cu"), 4, 2, 2, real="float"))
     for name, nv, k, s in (("euler_fp32.cu", 4, 2, 2), ("heat_fp32.cu", 1, 1, 1)):
         specs.append(standard_spec(eq_path(name), nv, k, s, 1024, 1024, real="float"))
         for b in ((16, 32) if nv > 1 else (16, 32, 64)):

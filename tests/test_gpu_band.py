@@ -1,10 +1,10 @@
-"""GPU parity of the band kernel (the swept solve of single-variable radius-1
-plugins with a point update, heat here) through the C ABI: against the
-oracle on ragged, odd and tiny grids (periodic wrap of rows and columns
-several times over), for every launch depth and row segment, FP64 and FP32
-state, forward Euler and RK2, with and without the full history.  Bitwise
-throughout: the point update evaluates the heat step's arithmetic in its
-order."""
+"""GPU parity of the band kernel (the swept solve of plugins with a point
+update, heat, or a band sweep of their own, Euler) through the C ABI:
+against the oracle on ragged, odd and tiny grids (periodic wrap of rows and
+columns several times over), for every launch depth and row segment, FP64
+and FP32 state, forward Euler and RK2, with and without the full history.
+Bitwise throughout: the point update and the band sweep evaluate the
+plugins' arithmetic in its order."""
 import numpy as np
 import pytest
 
@@ -98,7 +98,7 @@ def test_band_run_rejects_a_start_off_a_full_step(device):
     run, _ = _band(ic, g, 2, 4)
     with pytest.raises(hip.HipError, match="full-step"):
         hip.band_run(run.plan, 2, 4)
-    bad = hip.BandPlan(state=run.plan.state, nx=32, ny=32, its=2, T=run.T, levels=7, rows=8, store_all=0,
+    bad = hip.BandPlan(state=run.plan.state, nx=32, ny=32, its=2, T=run.T, levels=7, wout=114, rows=8, store_all=0,
                        band=run.plan.band)
     with pytest.raises(hip.HipError, match="levels"):
         hip.band_run(bad, 1, 4)
@@ -175,3 +175,98 @@ def test_band_fp32_state_equals_the_other_kernels(device):
     hip.synchronize()
     assert np.array_equal(sw.level(last), want)
     sw.close()
+
+
+# ------------------------------------------------------------------ Euler
+
+def _euler_band(ic, g, steps, rows=None, real="double", cu="euler.cu"):
+    from pysweep_amd import device as dev, hip, plugins, schedule
+    from pysweep_amd.equations import path as eq_path
+    nv, nx, ny = ic.shape
+    cpu = plugins.load_cpu_module(eq_path("euler.py"))
+    cpu.set_globals(*g)
+    run = dev.BandRun(eq_path(cu), 4, 2, 2, nx, ny, rows=rows, real=real)
+    run.set_globals(cpu, g)
+    run.upload_ic(ic)
+    last = run.run(schedule.standard_times(steps, 16, 2, 2))
+    hip.synchronize()
+    assert last == 1 + 2 * steps
+    return run, run.level(last)
+
+
+@pytest.mark.parametrize("nx,ny,rows", [(64, 64, 16), (48, 80, 7), (80, 48, 32), (20, 36, 128), (130, 70, 40)])
+def test_euler_band_matches_oracle(device, nx, ny, rows):
+    """The Euler plugin's band sweep (one column per lane, the radius-2
+    faces and pressures carried in registers) against the oracle bitwise on
+    grids whose sides are not multiples of the band or the row segment, and
+    on grids narrower than one band (columns wrap)."""
+    from oracle import euler as o_euler
+    steps = 6
+    dx = 10.0 / nx
+    dt = 0.1 * dx
+    X, Y = np.meshgrid(o_euler.grid(nx), np.linspace(-5, 5, ny, endpoint=False), indexing="ij")
+    ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=3.0)
+    g = [0.0, steps * dt, dt, dx, dx, 1.4]
+    want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
+    run, got = _euler_band(ic, g, steps, rows=rows)
+    assert np.array_equal(got, want)
+    assert not run.range_fault()
+    run.close()
+
+
+def test_euler_band_equals_standard_and_phases(device):
+    """256^2 vortex, 40 steps: band == standard == five phases (b=16), and
+    through the Solver (the default swept kernel) with the reference's step
+    quantization."""
+    from gpu_helpers import euler_case
+    ic, g, _ = euler_case(256, 40, eps=4.0, xc=1.0, yc=-1.5)
+    std = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
+    sw = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True, tuning={"phases": True})
+    run, got = _euler_band(ic, g, 40, rows=64)
+    assert np.array_equal(got, std.final_state)
+    assert np.array_equal(sw.final_state, std.final_state)
+    run.close()
+    bs = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=False)
+    from pysweep_amd import device as dev
+    assert isinstance(bs._run, dev.BandRun)
+    ps = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=False, tuning={"phases": True})
+    assert np.array_equal(bs.final_state, ps.final_state)
+
+
+def test_euler_band_fp32_equals_standard_fp32(device):
+    """FP32 state: the band sweep keeps each stage rounded to float in its
+    register rows, as the other kernels keep it in LDS and HBM."""
+    from gpu_helpers import euler_case
+    from pysweep_amd import device as dev, plugins
+    from pysweep_amd.equations import path as eq_path
+    n, steps = 192, 25
+    ic, g, _ = euler_case(n, steps, eps=4.0)
+    cpu = plugins.load_cpu_module(eq_path("euler.py"))
+    cpu.set_globals(*g)
+    st = dev.StandardRun(eq_path("euler_fp32.cu"), 4, 2, 2, n, n, real="float")
+    st.set_globals(cpu, g)
+    st.upload_ic(ic)
+    st.run(1, 2 * steps + 1)
+    want = st.level(2 * steps + 1)
+    st.close()
+    run, got = _euler_band(ic, g, steps, rows=48, real="float", cu="euler_fp32.cu")
+    assert np.array_equal(got, want)
+    run.close()
+
+
+def test_euler_band_flags_states_outside_the_division_range(device):
+    """The band sweep range-checks the cells it reads, as the other kernels
+    do: a state scaled by 2^-130 trips EU_RANGE_FAULT and the Solver solves
+    again with IEEE division."""
+    import warnings
+
+    from gpu_helpers import euler_case
+    ic, g, step = euler_case(64, 6)
+    small = ic * 2.0 ** -130
+    with warnings.catch_warnings(record=True):
+        warnings.simplefilter("always")
+        s = gpu_solve("euler", small, g, 2, 2, 16, swept=True)
+    assert s.division_rerun
+    assert np.array_equal(s.final_state, oracle_stages(step, small, 2, 12))
+    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=True)
+    assert not s2.division_rerun

@@ -36,11 +36,17 @@ def test_heat_matches_oracle(device, fe, swept, b, phases):
     assert np.array_equal(sol.final_state, want)
 
 
-@pytest.mark.parametrize("swept,b", [(False, 8), (True, 8), (True, 16), (True, 32)])
-def test_euler_matches_oracle(device, swept, b):
+@pytest.mark.parametrize("swept,b,phases", [(False, 8, False), (True, 16, False), (True, 8, True), (True, 16, True),
+                                            (True, 32, True)])
+def test_euler_matches_oracle(device, swept, b, phases):
+    """Euler swept solves run the band kernel (the plugin's band sweep) by
+    default and the five phase kernels with ``phases``; both, and the
+    standard kernel, give the oracle's bits."""
     n, steps = 64, 12
     ic, g, step = euler_case(n, steps, eps=5.0)
-    sol = gpu_solve("euler", ic, g, 2, 2, b, swept=swept, exact=True)
+    sol = gpu_solve("euler", ic, g, 2, 2, b, swept=swept, exact=True, tuning={"phases": phases})
+    from pysweep_amd import device as dev
+    assert isinstance(sol._run, dev.BandRun) == (swept and not phases)
     want = oracle_stages(step, ic, 2, 2 * steps)
     assert normwise(sol.final_state, want) <= TOL
     assert np.array_equal(sol.final_state, want)
@@ -97,10 +103,11 @@ def test_heat_swept_equals_standard_bitwise(device, b, phases):
     assert np.array_equal(a, c)
 
 
+@pytest.mark.parametrize("phases", [True, False], ids=["phases", "band"])
 @pytest.mark.parametrize("b", [8, 16, 32])
-def test_euler_swept_equals_standard_bitwise(device, b):
+def test_euler_swept_equals_standard_bitwise(device, b, phases):
     ic, g, _ = euler_case(128, 30, eps=3.0, xc=1.0, yc=-2.0)
-    a, c = _swept_vs_standard("euler", ic, g, 2, 2, b, 30)
+    a, c = _swept_vs_standard("euler", ic, g, 2, 2, b, 30, tuning={"phases": phases})
     assert np.array_equal(a, c)
 
 
@@ -113,9 +120,11 @@ def test_non_square_grid(device):
     ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=3.0)
     g = [0.0, steps * dt, dt, dx, dx, 1.4]
     sol = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True)
+    ph = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True, tuning={"phases": True})
     std = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
     want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
     assert np.array_equal(sol.final_state, want)
+    assert np.array_equal(ph.final_state, want)
     assert np.array_equal(std.final_state, want)
 
 
@@ -131,7 +140,8 @@ def test_tile_counts_not_multiple_of_xcds(device, nx, ny):
     ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=3.0)
     g = [0.0, steps * dt, dt, dx, dx, 1.4]
     want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
-    assert np.array_equal(gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True).final_state, want)
+    assert np.array_equal(gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True,
+                                    tuning={"phases": True}).final_state, want)
     assert np.array_equal(gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True).final_state, want)
 
 
@@ -223,8 +233,10 @@ def test_config_euler_1024_full_size(device):
     n, steps = 1024, 500
     ic, g, _ = euler_case(n, steps)
     sw = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True)
+    ph = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True, tuning={"phases": True})
     st = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
     assert np.array_equal(sw.final_state, st.final_state)
+    assert np.array_equal(ph.final_state, st.final_state)
     for v in range(4):
         assert abs(sw.final_state[v].sum() - ic[v].sum()) <= 1e-11 * np.abs(ic[v]).sum()
     X, Y = np.meshgrid(o_euler.grid(n), o_euler.grid(n), indexing="ij")
@@ -286,7 +298,8 @@ def test_large_heat_swept_equals_standard_and_decays(device, b, phases):
     assert abs(ratio - np.exp(-8 * np.pi ** 2 * steps * dt)) < 1e-9
 
 
-def test_large_euler_swept_equals_standard_conserves(device):
+@pytest.mark.parametrize("phases", [True, False], ids=["phases", "band"])
+def test_large_euler_swept_equals_standard_conserves(device, phases):
     """Euler 2048^2, b=32, 100 steps: swept == standard bitwise and the four
     integrals conserved to round-off."""
     from pysweep_amd import workloads
@@ -295,7 +308,7 @@ def test_large_euler_swept_equals_standard_conserves(device):
     dt = 0.1 * dx
     ic = workloads.vortex_state(n, n)
     g = [0.0, steps * dt, dt, dx, dx, 1.4]
-    sw = gpu_solve("euler", ic, g, 2, 2, 32, swept=True, exact=True)
+    sw = gpu_solve("euler", ic, g, 2, 2, 32, swept=True, exact=True, tuning={"phases": phases})
     st = gpu_solve("euler", ic, g, 2, 2, 32, swept=False, exact=True)
     assert np.array_equal(sw.final_state, st.final_state)
     for v in range(4):
@@ -335,9 +348,10 @@ def test_reference_block_sizes_euler(device, b):
     gives P=2, b=24 gives P=5): swept == standard == oracle, bitwise."""
     n, steps = 96 if b == 12 else 120, 15
     ic, g, step = euler_case(n, steps, eps=4.0)
-    sw = gpu_solve("euler", ic, g, 2, 2, b, swept=True, exact=True)
     want = oracle_stages(step, ic, 2, 2 * steps)
-    assert np.array_equal(sw.final_state, want)
+    for phases in (True, False):
+        sw = gpu_solve("euler", ic, g, 2, 2, b, swept=True, exact=True, tuning={"phases": phases})
+        assert np.array_equal(sw.final_state, want), phases
 
 
 def test_reference_heat_forward_euler_120_b24(device):
@@ -364,6 +378,8 @@ def test_reference_euler_vortex_120_b24(device):
     ic, g, step = euler_case(n, steps)
     sol = gpu_solve("euler", ic, g, 2, 2, 24, swept=True, exact=True)
     assert np.array_equal(sol.final_state, oracle_stages(step, ic, 2, 2 * steps))
+    ph = gpu_solve("euler", ic, g, 2, 2, 24, swept=True, exact=True, tuning={"phases": True})
+    assert np.array_equal(ph.final_state, sol.final_state)
     X, Y = np.meshgrid(o_euler.grid(n), o_euler.grid(n), indexing="ij")
     ex = o_euler.vortex(X, Y, steps * g[2], 1.4)
     assert np.abs(sol.final_state[0] - ex[0]).max() < 0.2
@@ -507,12 +523,12 @@ def test_face_prefetch_depth_heat(device, pfd):
 def test_face_prefetch_depth_euler(device, pfd):
     """The same for Euler 128^2, b=16 (6-level phases, fully unrolled)."""
     ic, g, step = euler_case(128, 20)
-    s = gpu_solve("euler", ic, g, 2, 2, 16, tuning={"pfd": pfd})
+    s = gpu_solve("euler", ic, g, 2, 2, 16, tuning={"pfd": pfd, "phases": True})
     assert np.array_equal(s.final_state, oracle_stages(step, ic, 2, 40))
 
 
-@pytest.mark.parametrize("swept", [True, False])
-def test_state_outside_the_division_range_is_solved_with_ieee_division(device, swept):
+@pytest.mark.parametrize("swept,phases", [(True, False), (True, True), (False, False)])
+def test_state_outside_the_division_range_is_solved_with_ieee_division(device, swept, phases):
     """An Euler state scaled by 2^-130 (rho, E and p below the range in which
     the plugin's fast quotients are the IEEE ones) trips EU_RANGE_FAULT; the
     Solver solves again with IEEE division and square root and gives the
@@ -522,10 +538,10 @@ def test_state_outside_the_division_range_is_solved_with_ieee_division(device, s
     small = ic * 2.0 ** -130
     with warnings.catch_warnings(record=True) as w:
         warnings.simplefilter("always")
-        s = gpu_solve("euler", small, g, 2, 2, 16, swept=swept)
+        s = gpu_solve("euler", small, g, 2, 2, 16, swept=swept, tuning={"phases": phases})
     assert s.division_rerun and any("division" in str(x.message) for x in w)
     assert np.array_equal(s.final_state, oracle_stages(step, small, 2, 12))
-    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=swept)
+    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=swept, tuning={"phases": phases})
     assert not s2.division_rerun
 
 
