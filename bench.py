@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_euler1024_b16.json")
 #: the kernel build the committed counter file was measured on (bumped when
 #: the swept or standard kernels change)
-KERNEL_BUILD = 4
+KERNEL_BUILD = 5
 
 
 def parse():
