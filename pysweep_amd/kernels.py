@@ -190,7 +190,7 @@ def band_rows(spec, nx, ny, simds=BAND_SIMDS):
     H, k = spec.levels, spec.ops
     point = has_point_update(spec.user_path)
     best = None
-    for rows in range(4, min(nx, 256) + 1):
+    for rows in range(min(4, nx), min(nx, 256) + 1):
         waves = bands * -(-nx // rows)
         steps = 3 * -(-(rows + 2 * k * H) // 3)
         if point:

@@ -10,6 +10,12 @@ def solver_print(solver):
     """The verbose summary of a solve (the reference prints its own banner,
     core/io.py:102-132; this one is laid out for one GPU)."""
     method = "swept" if solver.simulation else "standard"
+    run = getattr(solver, "_run", None)
+    spec = getattr(run, "spec", None)
+    if solver.simulation and getattr(spec, "kind", None) == "band":
+        method += f" (band kernel, {spec.levels} levels per launch, {run.rows} rows per wavefront)"
+    elif solver.simulation and spec is not None:
+        method += " (phase kernels)"
     rows = [("method", f"{method}, blocksize {solver.blocksize[0]}, {solver.operating} operating points "
                        f"({2 * solver.operating + 1}-point cross), {solver.intermediate} intermediate steps"),
             ("placement", f"share {solver.share} of the block rows on the GPU, the rest on the host"

@@ -43,17 +43,19 @@ def _band(ic, g, its, steps, levels=None, rows=None, real="double", cu="heat.cu"
     return run, out
 
 
+@pytest.mark.parametrize("rows", [48, None], ids=["rows48", "rows_auto"])
 @pytest.mark.parametrize("fe", [True, False], ids=["fe", "rk2"])
-@pytest.mark.parametrize("nx,ny", [(64, 96), (130, 66), (37, 45), (3, 200), (200, 3), (5, 7), (96, 257)])
-def test_band_matches_oracle_on_ragged_and_tiny_grids(device, nx, ny, fe):
+@pytest.mark.parametrize("nx,ny", [(64, 96), (130, 66), (37, 45), (3, 200), (200, 3), (5, 7), (96, 257), (1, 2)])
+def test_band_matches_oracle_on_ragged_and_tiny_grids(device, nx, ny, fe, rows):
     """Odd ny (no paired loads), ny below one band (columns wrap several
-    times), nx below the halo (rows wrap several times), bands and row
-    segments that do not divide the grid; 13 steps, so the last launch is
-    shorter than the others."""
+    times), nx below the halo (rows wrap several times, down to one row),
+    bands and row segments that do not divide the grid, with a fixed segment
+    and the model's; 13 steps, so the last launch is shorter than the
+    others."""
     its = 1 if fe else 2
     steps = 13
     ic, g, step = _heat_grid(nx, ny, fe, steps, seed=nx + ny)
-    run, got = _band(ic, g, its, steps, rows=48)
+    run, got = _band(ic, g, its, steps, rows=rows)
     assert np.array_equal(got, oracle_stages(step, ic, its, its * steps))
     run.close()
 
@@ -194,7 +196,8 @@ def _euler_band(ic, g, steps, rows=None, real="double", cu="euler.cu"):
     return run, run.level(last)
 
 
-@pytest.mark.parametrize("nx,ny,rows", [(64, 64, 16), (48, 80, 7), (80, 48, 32), (20, 36, 128), (130, 70, 40)])
+@pytest.mark.parametrize("nx,ny,rows", [(64, 64, 16), (48, 80, 7), (80, 48, 32), (20, 36, 128), (130, 70, 40),
+                                         (3, 40, None), (36, 20, None), (130, 70, None)])
 def test_euler_band_matches_oracle(device, nx, ny, rows):
     """The Euler plugin's band sweep (one column per lane, the radius-2
     faces and pressures carried in registers) against the oracle bitwise on

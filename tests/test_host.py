@@ -515,3 +515,45 @@ def test_tuner_records_and_shapes(tmp_path):
                 assert (ssp.bx, ssp.by, ssp.threads) == (*cand["stile"], cand["sthreads"])
             if "slots" in cand:
                 assert sp.slots == cand["slots"]
+
+
+def test_verbose_summary_names_the_swept_kernel():
+    """The solve summary says which swept kernel ran (no GPU needed: a
+    stand-in run object carries the module spec)."""
+    import types
+
+    from pysweep_amd import Solver, reporting
+    s = Solver(np.zeros((4, 64, 64)), sendWarning=False)
+    s.globals = [0.0, 0.1, 0.01, 0.1, 0.1, 1.4]
+    s.cpuStr, s.gpuStr = "euler.py", "euler.cu"
+    assert "swept, blocksize" in reporting.solver_print(s)
+    spec = kernels.band_spec(eq_path("euler.cu"), 4, 2, 2)
+    s._run = types.SimpleNamespace(spec=spec, rows=10)
+    assert "band kernel, 2 levels per launch, 10 rows per wavefront" in reporting.solver_print(s)
+    s._run = types.SimpleNamespace(spec=kernels.swept_spec(eq_path("euler.cu"), 4, 2, 2, 16), rows=None)
+    assert "(phase kernels)" in reporting.solver_print(s)
+
+
+def test_band_forms_and_launch_shapes():
+    """Which plugins have a band form, their bands' output columns, and the
+    row segments the wave-balance model picks for the measured workloads
+    (Euler 1024^2 and 2048^2, heat 8192^2: the segments the MI355X
+    measurements found best or within 1 % of it)."""
+    heat, euler = eq_path("heat.cu"), eq_path("euler.cu")
+    assert kernels.band_capable(heat, 1, 1, 1) and kernels.band_capable(heat, 1, 1, 2)
+    assert kernels.band_capable(euler, 4, 2, 2)
+    assert not kernels.band_capable(eq_path("example.cu"), 1, 1, 1)
+    h = kernels.band_spec(heat, 1, 1, 1)
+    e = kernels.band_spec(euler, 4, 2, 2)
+    assert (h.levels, kernels.band_out_columns(h)) == (12, 104)
+    assert (e.levels, kernels.band_out_columns(e)) == (2, 56)
+    assert kernels.band_spec(eq_path("heat_fp32.cu"), 1, 1, 1, real="float").levels == 8
+    assert kernels.band_rows(e, 1024, 1024) == 10
+    assert kernels.band_rows(e, 2048, 2048) == 40
+    assert kernels.band_rows(h, 8192, 8192) == 216
+    for n in (3, 5, 64, 130):
+        assert 1 <= kernels.band_rows(e, n, n) <= max(n, 4)
+    with pytest.raises(ValueError):
+        kernels.band_spec(euler, 4, 2, 2, levels=16)  # 64 - 2*2*16 < 8 output columns
+    src = h.source()
+    assert src.index("ps_band_grid") < src.index("ps_point") < src.index("Band(PS_REAL")
