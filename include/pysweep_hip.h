@@ -262,7 +262,7 @@ typedef struct ps_band_plan {
     void *state;    /* ring state[T][nx][ny]                               */
     int nx, ny;
     int its;        /* intermediate steps s (1 or 2)                       */
-    int T;          /* at least levels + s                                 */
+    int T;          /* at least levels + 1 (a launch reads level A only)   */
     int levels;     /* levels per launch compiled into the kernel (even,
                        a multiple of s, 2..16)                              */
     int wout;       /* output columns per band (the kernel's band width less

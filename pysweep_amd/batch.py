@@ -174,6 +174,21 @@ class PluginClass:
         self.cpu = plugins.load_cpu_module(eq_path(py), name=f"pysweep_batch_{plugin}")
         self.key = (plugin, precision, its, b)
         self._solo = None
+        self._band = None
+        self.cu = eq_path(cu64 if precision == "fp64" else cu32)
+
+    def band_module(self, nx, ny):
+        """The class's band module, loaded on first use, when the band kernel
+        outruns the standard kernel on an nx x ny grid (a problem running
+        alone then takes its stages on it); else None."""
+        if not kernels.band_beats_standard(self.cu, self.nv, self.ops, self.its, nx, ny):
+            return None
+        if self._band is None:
+            spec = kernels.band_spec(self.cu, self.nv, self.ops, self.its, real=self.real)
+            m = kernels.load(spec)
+            self.modules.append(m)
+            self._band = (spec, m, m.get_function("Band"))
+        return self._band
 
     def levels(self, method):
         """Ring levels of a problem: the swept window 2P+s (core/solver.py:123),
@@ -547,6 +562,12 @@ class Engine:
     its initial condition goes in (a read of a cell that no phase wrote then
     turns the result into NaN).
 
+    ``band_solo``: a problem running alone takes its standard stages on the
+    band kernel where that outruns the standard kernel (heat at every size,
+    Euler from 1024^2 cells: ``kernels.band_beats_standard``), in whole RK2
+    steps, the stages before the first full step on the standard kernel;
+    results are bitwise unchanged.
+
     ``tuner``: the ``autotune.Tuner`` whose records give each class its
     block size, the method of problems submitted with method="auto" and the
     workgroup shapes of its kernels; "default" loads the stored records
@@ -555,7 +576,8 @@ class Engine:
 
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, tuner="default", clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
-                 capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4, poison=False):
+                 capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4, poison=False,
+                 band_solo=True):
         hip.ensure_init(0)
         self.policy = policy
         self.admit_bytes = admit_bytes  # initial-condition bytes staged per step at most
@@ -563,6 +585,10 @@ class Engine:
         self._throttled = False
         self.capacity = capacity  # cells per round for "oldest_first" and "short_first"
         self.solo = solo  # a problem running alone uses the plain kernels
+        # ... and the band kernel for its standard stages where that outruns
+        # the standard kernel (kernels.band_beats_standard)
+        self.band_solo = band_solo
+        self.band_stages = 0  # stages run on the band kernel (solo problems)
         # host steps whose rounds may be queued on the GPU at once: admitting
         # a request waits for the queued rounds, so this bounds its delay
         self.max_inflight = max_inflight
@@ -726,12 +752,15 @@ class Engine:
         lookups."""
         cls = t.cls
         sw, st, msw, mst, f, fstd = cls.solo_modules(t.nx, t.ny)
+        band = cls.band_module(t.nx, t.ny) if self.band_solo else None
+        mods = [msw, mst] + ([band[1]] if band else [])
         # Modules are shared by every class, engine and Solver that load the
         # same spec (kernels.load), so the owner of their constants is kept
         # on the modules themselves; any constant write clears it.
-        if msw.const_owner is not t or mst.const_owner is not t:
-            cls.cpu.set_globals(*t.problem.globals, source_mod=plugins.SourceModuleFacade([msw, mst]))
-            msw.const_owner = mst.const_owner = t
+        if any(m.const_owner is not t for m in mods):
+            cls.cpu.set_globals(*t.problem.globals, source_mod=plugins.SourceModuleFacade(mods))
+            for m in mods:
+                m.const_owner = t
         G, P, T = t.times.G, cls.P, t.T
         c0, c1 = t.c, t.c + R
         if G >= 0 and c0 <= G + 1:
@@ -743,9 +772,24 @@ class Engine:
         if c1 > lo:
             done = G + 1
             base = done * P if G >= 0 else 0
+            g0, g1 = base + lo - done, base + c1 - done
             plan = hip.StandardPlan(state=t.ring.ptr, nx=t.nx, ny=t.ny, its=cls.its, T=T, bx=st.bx, by=st.by,
                                     threads=st.threads, standard=fstd)
-            hip.standard_run(plan, base + lo - done, base + c1 - done)
+            if band is not None and T >= band[0].levels + 1:
+                # the band kernel starts on a full step: stage g reads level
+                # s-2+g, a full step when (g - 1) % s == 0
+                while (g0 - 1) % cls.its and g0 < g1:
+                    hip.standard_run(plan, g0, g0 + 1)
+                    g0 += 1
+                if g0 < g1:
+                    spec = band[0]
+                    bplan = hip.BandPlan(state=t.ring.ptr, nx=t.nx, ny=t.ny, its=cls.its, T=T, levels=spec.levels,
+                                         wout=kernels.band_out_columns(spec),
+                                         rows=kernels.band_rows(spec, t.nx, t.ny), store_all=0, band=band[2])
+                    hip.band_run(bplan, cls.its - 2 + g0, g1 - g0)
+                    self.band_stages += g1 - g0
+            else:
+                hip.standard_run(plan, g0, g1)
 
     def _track(self, R):
         """Bound the queued GPU work (see ``max_inflight``) and keep the

@@ -757,7 +757,7 @@ static int band_run(const ps_band_plan *p, int A, int stages)
     const int H = p->levels, s = p->its;
     if (s < 1 || s > 2) return fail("band solve: intermediate steps must be 1 or 2");
     if (H < 2 || H > 16 || H % 2 || H % s) return fail("band solve: levels must be even, a multiple of s, 2..16");
-    if (p->T < H + s) return fail("band ring must hold at least levels + s levels");
+    if (p->T < H + 1) return fail("band ring must hold at least levels + 1 levels");
     if (p->nx < 1 || p->ny < 1 || p->rows < 1) return fail("band solve: empty grid or row segment");
     if (A < s - 1 || (A - (s - 1)) % s) return fail("band solve: must start on a full-step level");
     if (stages < 0) return fail("band solve: negative stage count");

@@ -157,6 +157,36 @@ def band_capable(user_path, nv, ops, its):
     return has_band_sweep(user_path)
 
 
+#: cells from which a plugin's own band sweep (Euler: one column per lane,
+#: a 2-column halo per level) is the faster swept kernel: its 64-column
+#: bands leave small grids too few wavefronts.  Measured on MI355X (Euler,
+#: GCUPS, band / phase kernels b=16): 128^2 0.67 / 0.83, 256^2 2.64 / 3.20,
+#: 384^2 5.81 / 4.82, 512^2 7.55 / 7.52, 1024^2 14.41 / 9.73.  Heat's band
+#: kernel led at every size measured (64^2 to 8192^2).
+BAND_SWEEP_MIN_CELLS = 1 << 17
+#: cells from which the Euler band sweep outruns the standard kernel
+#: (GCUPS, band / standard: 512^2 7.60 / 10.22, 768^2 11.42 / 12.42,
+#: 1024^2 14.57 / 13.77, 1536^2 17.86 / 14.89); heat's band kernel outruns
+#: it at every size (64^2 2.98 / 1.09 .. 2048^2 1179 / 286).
+BAND_OVER_STANDARD_MIN_CELLS = 1 << 20
+
+
+def band_default(user_path, nv, ops, its, nx, ny):
+    """Whether a swept solve of this plugin and grid runs on the band kernel
+    by default (else on the phase kernels)."""
+    if not band_capable(user_path, nv, ops, its):
+        return False
+    return has_point_update(user_path) or nx * ny >= BAND_SWEEP_MIN_CELLS
+
+
+def band_beats_standard(user_path, nv, ops, its, nx, ny):
+    """Whether the band kernel outruns the standard kernel on this grid (the
+    engine then runs a problem alone on it)."""
+    if not band_capable(user_path, nv, ops, its):
+        return False
+    return has_point_update(user_path) or nx * ny >= BAND_OVER_STANDARD_MIN_CELLS
+
+
 def band_out_columns(spec):
     """Output columns per band of a band module: a point-update band is 128
     columns (two per lane) less one halo column per level on each side; a

@@ -19,9 +19,10 @@ quantization to multiples of the pyramid height), ``history`` (keep every
 full-step level, the reference's output dataset ``data``), ``result`` (final
 full-step state), ``solve_seconds`` and ``gcups``; ``tuning`` (launch shape:
 slots / threads / bx / by / pfd; band_levels / band_rows for the band
-kernel, which runs the swept solve of a plugin with a point update (heat) or
-a band sweep of its own (Euler), and ``phases=True`` to run it on the five
-phase kernels instead; ``generic=True`` to call the plugin's
+kernel, which runs the swept solve of a plugin with a point update (heat),
+or with a band sweep of its own (Euler) on grids of at least 2^17 cells;
+``band=True`` or ``phases=True`` to run it on the band kernel or the five
+phase kernels regardless; ``generic=True`` to call the plugin's
 per-cell step even when it provides a cooperative level update, and
 ``exact_division=True`` to build the plugin with EU_EXACT_DIVISION, and
 ``poison=True`` to fill the device ring with NaN before the initial
@@ -267,8 +268,14 @@ class Solver(object):
         hip.ensure_init(0)
         defines = ("EU_EXACT_DIVISION",) if t.get("exact_division") else ()
         from pysweep_amd import kernels
-        band = self.simulation and not t.get("phases") and not t.get("generic") and \
+        capable = self.simulation and not t.get("generic") and \
             kernels.band_capable(self.gpu, nv, self.operating, self.intermediate)
+        if t.get("phases") or not capable:
+            band = False
+        elif t.get("band"):
+            band = True
+        else:
+            band = kernels.band_default(self.gpu, nv, self.operating, self.intermediate, nx, ny)
         if band:
             self._run = device.BandRun(self.gpu, nv, self.operating, self.intermediate, nx, ny,
                                        levels=t.get("band_levels"), rows=t.get("band_rows"), defines=defines)

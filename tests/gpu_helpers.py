@@ -28,6 +28,12 @@ def gpu_solve(name, ic, globals_, k, s, b, swept=True, exact=True, history=False
     return solver
 
 
+def swept_kernel(phases):
+    """Solver tuning that runs a swept solve on the five phase kernels
+    (phases=True) or on the band kernel, whatever the grid size."""
+    return {"phases": True} if phases else {"band": True}
+
+
 def heat_case(n, fe, steps, seed=0):
     dx = 1.0 / n
     dt = 0.1 * dx * dx
@@ -58,4 +64,4 @@ def normwise(a, b):
 
 
 __all__ = ["gpu_solve", "heat_case", "euler_case", "oracle_stages", "normwise", "o_plugins", "o_swept",
-           "o_euler", "o_heat"]
+           "o_euler", "o_heat", "swept_kernel"]

@@ -8,7 +8,7 @@ plugins' arithmetic in its order."""
 import numpy as np
 import pytest
 
-from gpu_helpers import gpu_solve, heat_case, o_heat, oracle_stages
+from gpu_helpers import gpu_solve, heat_case, o_heat, oracle_stages, swept_kernel
 
 pytestmark = pytest.mark.gpu
 
@@ -219,8 +219,7 @@ def test_euler_band_matches_oracle(device, nx, ny, rows):
 
 def test_euler_band_equals_standard_and_phases(device):
     """256^2 vortex, 40 steps: band == standard == five phases (b=16), and
-    through the Solver (the default swept kernel) with the reference's step
-    quantization."""
+    through the Solver with the reference's step quantization."""
     from gpu_helpers import euler_case
     ic, g, _ = euler_case(256, 40, eps=4.0, xc=1.0, yc=-1.5)
     std = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
@@ -229,7 +228,7 @@ def test_euler_band_equals_standard_and_phases(device):
     assert np.array_equal(got, std.final_state)
     assert np.array_equal(sw.final_state, std.final_state)
     run.close()
-    bs = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=False)
+    bs = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=False, tuning=swept_kernel(False))
     from pysweep_amd import device as dev
     assert isinstance(bs._run, dev.BandRun)
     ps = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=False, tuning={"phases": True})
@@ -268,8 +267,54 @@ def test_euler_band_flags_states_outside_the_division_range(device):
     small = ic * 2.0 ** -130
     with warnings.catch_warnings(record=True):
         warnings.simplefilter("always")
-        s = gpu_solve("euler", small, g, 2, 2, 16, swept=True)
+        s = gpu_solve("euler", small, g, 2, 2, 16, swept=True, tuning=swept_kernel(False))
     assert s.division_rerun
     assert np.array_equal(s.final_state, oracle_stages(step, small, 2, 12))
-    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=True)
+    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, tuning=swept_kernel(False))
     assert not s2.division_rerun
+
+
+def test_solver_picks_the_band_kernel_by_plugin_and_size(device):
+    """Heat's swept solves take the band kernel at every size; Euler's from
+    2^17 cells (below, the phase kernels: the band's 64-column bands leave
+    small grids too few wavefronts)."""
+    from gpu_helpers import euler_case
+    from pysweep_amd import device as dev
+    ic, g, _ = heat_case(64, True, 5)
+    assert isinstance(gpu_solve("heat", ic, g, 1, 1, 8)._run, dev.BandRun)
+    ic, g, _ = euler_case(256, 2)
+    assert isinstance(gpu_solve("euler", ic, g, 2, 2, 16)._run, dev.SweptRun)
+    ic, g, _ = euler_case(384, 2)
+    assert isinstance(gpu_solve("euler", ic, g, 2, 2, 16)._run, dev.BandRun)
+
+
+def test_engine_runs_lone_problems_on_the_band_kernel(device):
+    """A problem running alone in the engine takes its standard stages on
+    the band kernel where that outruns the standard kernel (Euler from
+    1024^2 cells) and its ring holds a launch's levels: an Euler 1024^2
+    solve in one run of rounds, and again in runs of three rounds, so that
+    every other run starts on a half step (one standard stage realigns it,
+    and a run may end on one).  Results equal the plain solves bitwise;
+    heat problems, whose band launches are 12 levels deep, keep the
+    standard kernel in rings of s+1 levels."""
+    from gpu_helpers import euler_case
+    from pysweep_amd import baseline
+    from pysweep_amd.batch import Engine, Problem
+    ic, g, _ = euler_case(1024, 10)
+    plain = None
+    for rounds, lo, hi in ((1 << 20, 18, 20), (3, 1, 19)):
+        eng = Engine(tuner=None, max_rounds_ahead=rounds)
+        t = eng.submit(Problem("euler", ic, g, 2, method="standard"), arrival=0.0)
+        eng.run()
+        assert lo <= eng.band_stages <= hi, eng.band_stages
+        plain = baseline.solve_plain(t.problem) if plain is None else plain
+        assert np.array_equal(t.result, plain)
+    eng = Engine(tuner=None, band_solo=False)
+    t = eng.submit(Problem("euler", ic, g, 2, method="standard"), arrival=0.0)
+    eng.run()
+    assert eng.band_stages == 0 and np.array_equal(t.result, plain)
+    h_ic, h_g, h_step = heat_case(64, False, 10, seed=2)
+    eng = Engine(tuner=None)
+    th = eng.submit(Problem("heat", h_ic, h_g, 2, method="standard"), arrival=0.0)
+    eng.run()
+    assert eng.band_stages == 0 and np.array_equal(th.result, oracle_stages(h_step, h_ic, 2, 20))

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from gpu_helpers import (euler_case, gpu_solve, heat_case, normwise, o_euler, o_heat, o_plugins, o_swept,
-                         oracle_stages)
+                         oracle_stages, swept_kernel)
 
 pytestmark = pytest.mark.gpu
 TOL = 1e-12
@@ -22,13 +22,12 @@ TOL = 1e-12
 @pytest.mark.parametrize("swept,b,phases", [(False, 8, False), (True, 8, False), (True, 32, False), (True, 8, True),
                                             (True, 16, True), (True, 32, True)])
 def test_heat_matches_oracle(device, fe, swept, b, phases):
-    """Heat swept solves run the band kernel by default and the five phase
-    kernels with ``phases``; both, and the standard kernel, give the
-    oracle's bits."""
+    """Heat swept solves on the band kernel (their default) and on the five
+    phase kernels; both, and the standard kernel, give the oracle's bits."""
     n, steps = 64, 40
     s = 1 if fe else 2
     ic, g, step = heat_case(n, fe, steps, seed=1)
-    sol = gpu_solve("heat", ic, g, 1, s, b, swept=swept, exact=True, tuning={"phases": phases})
+    sol = gpu_solve("heat", ic, g, 1, s, b, swept=swept, exact=True, tuning=swept_kernel(phases))
     from pysweep_amd import device as dev
     assert isinstance(sol._run, dev.BandRun) == (swept and not phases)
     want = oracle_stages(step, ic, s, s * steps)
@@ -39,12 +38,12 @@ def test_heat_matches_oracle(device, fe, swept, b, phases):
 @pytest.mark.parametrize("swept,b,phases", [(False, 8, False), (True, 16, False), (True, 8, True), (True, 16, True),
                                             (True, 32, True)])
 def test_euler_matches_oracle(device, swept, b, phases):
-    """Euler swept solves run the band kernel (the plugin's band sweep) by
-    default and the five phase kernels with ``phases``; both, and the
-    standard kernel, give the oracle's bits."""
+    """Euler swept solves on the band kernel (the plugin's band sweep; the
+    default from 2^17 cells, asked for here) and on the five phase kernels;
+    both, and the standard kernel, give the oracle's bits."""
     n, steps = 64, 12
     ic, g, step = euler_case(n, steps, eps=5.0)
-    sol = gpu_solve("euler", ic, g, 2, 2, b, swept=swept, exact=True, tuning={"phases": phases})
+    sol = gpu_solve("euler", ic, g, 2, 2, b, swept=swept, exact=True, tuning=swept_kernel(phases))
     from pysweep_amd import device as dev
     assert isinstance(sol._run, dev.BandRun) == (swept and not phases)
     want = oracle_stages(step, ic, 2, 2 * steps)
@@ -99,7 +98,7 @@ def _swept_vs_standard(name, ic, g, k, s, b, steps, tuning=None):
 @pytest.mark.parametrize("b", [8, 16, 32])
 def test_heat_swept_equals_standard_bitwise(device, b, phases):
     ic, g, _ = heat_case(128, True, 100, seed=2)
-    a, c = _swept_vs_standard("heat", ic, g, 1, 1, b, 100, tuning={"phases": phases})
+    a, c = _swept_vs_standard("heat", ic, g, 1, 1, b, 100, tuning=swept_kernel(phases))
     assert np.array_equal(a, c)
 
 
@@ -107,7 +106,7 @@ def test_heat_swept_equals_standard_bitwise(device, b, phases):
 @pytest.mark.parametrize("b", [8, 16, 32])
 def test_euler_swept_equals_standard_bitwise(device, b, phases):
     ic, g, _ = euler_case(128, 30, eps=3.0, xc=1.0, yc=-2.0)
-    a, c = _swept_vs_standard("euler", ic, g, 2, 2, b, 30, tuning={"phases": phases})
+    a, c = _swept_vs_standard("euler", ic, g, 2, 2, b, 30, tuning=swept_kernel(phases))
     assert np.array_equal(a, c)
 
 
@@ -119,7 +118,7 @@ def test_non_square_grid(device):
     X, Y = np.meshgrid(o_euler.grid(nx), np.linspace(-5, 5, ny, endpoint=False), indexing="ij")
     ic = o_euler.vortex(X, Y, 0.0, 1.4, eps=3.0)
     g = [0.0, steps * dt, dt, dx, dx, 1.4]
-    sol = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True)
+    sol = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True, tuning={"band": True})
     ph = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=True, tuning={"phases": True})
     std = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True)
     want = oracle_stages(lambda q, qb, gg: o_euler.step_py(q, qb, gg, 1.4, dt, dx, dx), ic, 2, 2 * steps)
@@ -211,7 +210,7 @@ def test_config_heat_384_swept_cpu_case(device, phases):
     and the analytic decay of the sin(2 pi x) sin(2 pi y) mode."""
     n, steps = 384, 100
     ic, g, step = heat_case(n, True, steps, seed=0)
-    sol = gpu_solve("heat", ic, g, 1, 1, 8, swept=True, exact=False, tuning={"phases": phases})
+    sol = gpu_solve("heat", ic, g, 1, 1, 8, swept=True, exact=False, tuning=swept_kernel(phases))
     assert sol.times.stages == 99 and sol.times.G == 32
     want = oracle_stages(step, ic, 1, 99)
     assert np.array_equal(sol.final_state, want)
@@ -291,7 +290,7 @@ def test_large_heat_swept_equals_standard_and_decays(device, b, phases):
     dt = 0.1 * dx * dx
     ic = workloads.heat_mode(n, n)
     g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
-    sw = gpu_solve("heat", ic, g, 1, 1, b, swept=True, exact=True, tuning={"phases": phases})
+    sw = gpu_solve("heat", ic, g, 1, 1, b, swept=True, exact=True, tuning=swept_kernel(phases))
     st = gpu_solve("heat", ic, g, 1, 1, b, swept=False, exact=True)
     assert np.array_equal(sw.final_state, st.final_state)
     ratio = sw.final_state[0, n // 4, n // 4] / ic[0, n // 4, n // 4]
@@ -308,7 +307,7 @@ def test_large_euler_swept_equals_standard_conserves(device, phases):
     dt = 0.1 * dx
     ic = workloads.vortex_state(n, n)
     g = [0.0, steps * dt, dt, dx, dx, 1.4]
-    sw = gpu_solve("euler", ic, g, 2, 2, 32, swept=True, exact=True, tuning={"phases": phases})
+    sw = gpu_solve("euler", ic, g, 2, 2, 32, swept=True, exact=True, tuning=swept_kernel(phases))
     st = gpu_solve("euler", ic, g, 2, 2, 32, swept=False, exact=True)
     assert np.array_equal(sw.final_state, st.final_state)
     for v in range(4):
@@ -337,7 +336,7 @@ def test_reference_sweep_heat_rk2_384(device, b, swept):
     step = lambda q, qb, gg: o_heat.step_py(q[0], qb[0], gg, c, False)[None]
     want = oracle_stages(step, ic, 2, 2 * steps)
     for phases in ((True, False) if swept else (False,)):
-        sol = gpu_solve("heat", ic, g, 1, 2, b, swept=swept, exact=True, tuning={"phases": phases})
+        sol = gpu_solve("heat", ic, g, 1, 2, b, swept=swept, exact=True, tuning=swept_kernel(phases))
         assert np.array_equal(sol.final_state, want)
         assert np.allclose(sol.final_state[0], ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt))
 
@@ -350,7 +349,7 @@ def test_reference_block_sizes_euler(device, b):
     ic, g, step = euler_case(n, steps, eps=4.0)
     want = oracle_stages(step, ic, 2, 2 * steps)
     for phases in (True, False):
-        sw = gpu_solve("euler", ic, g, 2, 2, b, swept=True, exact=True, tuning={"phases": phases})
+        sw = gpu_solve("euler", ic, g, 2, 2, b, swept=True, exact=True, tuning=swept_kernel(phases))
         assert np.array_equal(sw.final_state, want), phases
 
 
@@ -366,7 +365,7 @@ def test_reference_heat_forward_euler_120_b24(device):
     g = [0.0, steps * dt, dt, dx, dx, 1.0, True]
     exact = ic[0] * np.exp(-8 * np.pi ** 2 * steps * dt)
     for phases in (True, False):
-        sol = gpu_solve("heat", ic, g, 1, 1, 24, swept=True, exact=True, tuning={"phases": phases})
+        sol = gpu_solve("heat", ic, g, 1, 1, 24, swept=True, exact=True, tuning=swept_kernel(phases))
         assert np.allclose(sol.final_state[0], exact)
 
 
@@ -376,7 +375,7 @@ def test_reference_euler_vortex_120_b24(device):
     error (the reference prints its maximum error)."""
     n, steps = 120, 200
     ic, g, step = euler_case(n, steps)
-    sol = gpu_solve("euler", ic, g, 2, 2, 24, swept=True, exact=True)
+    sol = gpu_solve("euler", ic, g, 2, 2, 24, swept=True, exact=True, tuning={"band": True})
     assert np.array_equal(sol.final_state, oracle_stages(step, ic, 2, 2 * steps))
     ph = gpu_solve("euler", ic, g, 2, 2, 24, swept=True, exact=True, tuning={"phases": True})
     assert np.array_equal(ph.final_state, sol.final_state)
@@ -538,10 +537,10 @@ def test_state_outside_the_division_range_is_solved_with_ieee_division(device, s
     small = ic * 2.0 ** -130
     with warnings.catch_warnings(record=True) as w:
         warnings.simplefilter("always")
-        s = gpu_solve("euler", small, g, 2, 2, 16, swept=swept, tuning={"phases": phases})
+        s = gpu_solve("euler", small, g, 2, 2, 16, swept=swept, tuning=swept_kernel(phases))
     assert s.division_rerun and any("division" in str(x.message) for x in w)
     assert np.array_equal(s.final_state, oracle_stages(step, small, 2, 12))
-    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=swept, tuning={"phases": phases})
+    s2 = gpu_solve("euler", ic, g, 2, 2, 16, swept=swept, tuning=swept_kernel(phases))
     assert not s2.division_rerun
 
 
@@ -579,7 +578,7 @@ def test_nan_poisoned_ring_swept_equals_oracle(device, name, b, k, s, fe):
         stages = 24
     want = oracle_stages(step, ic, s, stages)
     for swept, phases in ((True, True), (True, False), (False, False)):
-        r = gpu_solve(name, ic, g, k, s, b, swept=swept, tuning={"poison": True, "phases": phases})
+        r = gpu_solve(name, ic, g, k, s, b, swept=swept, tuning={"poison": True, **swept_kernel(phases)})
         assert np.isfinite(r.final_state).all()
         assert np.array_equal(r.final_state, want), (swept, phases)
 
