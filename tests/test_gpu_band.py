@@ -318,3 +318,20 @@ def test_engine_runs_lone_problems_on_the_band_kernel(device):
     th = eng.submit(Problem("heat", h_ic, h_g, 2, method="standard"), arrival=0.0)
     eng.run()
     assert eng.band_stages == 0 and np.array_equal(th.result, oracle_stages(h_step, h_ic, 2, 20))
+
+
+def test_euler_history_through_the_band_kernel(device):
+    """The Solver's history of an Euler solve on the band kernel (384^2, the
+    default there): every full step, written by launches that record every
+    level, equals the standard solve's, with the reference's quantization
+    (an odd stage count: the last launch takes one stage)."""
+    from gpu_helpers import euler_case
+    from pysweep_amd import device as dev
+    n, steps = 384, 8
+    ic, g, _ = euler_case(n, steps, eps=4.0)
+    sw = gpu_solve("euler", ic, g, 2, 2, 16, swept=True, exact=False, history=True)
+    assert isinstance(sw._run, dev.BandRun) and sw.times.stages == 15
+    st = gpu_solve("euler", ic, g, 2, 2, 16, swept=False, exact=True, history=True)
+    assert sw.data.shape[0] == 8
+    for m in range(sw.data.shape[0]):
+        assert np.array_equal(sw.data[m], st.data[m]), m
