@@ -270,6 +270,9 @@ class Solver(object):
         from pysweep_amd import kernels
         capable = self.simulation and not t.get("generic") and \
             kernels.band_capable(self.gpu, nv, self.operating, self.intermediate)
+        if t.get("band") and not t.get("phases") and self.simulation and not capable:
+            warnings.warn("tuning band=True: this plugin (or the per-cell step contract) has no band form; "
+                          "the swept solve runs on the phase kernels")
         if t.get("phases") or not capable:
             band = False
         elif t.get("band"):
