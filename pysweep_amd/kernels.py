@@ -529,6 +529,7 @@ def shipped_module_specs():
         specs.append(band_spec(heat, 1, 1, s_))
     specs.append(band_spec(eq_path("heat_fp32.cu"), 1, 1, 1, real="float"))
     specs.append(band_spec(euler, 4, 2, 2))
+    specs.append(band_spec(euler, 4, 2, 2, defines=("EU_EXACT_DIVISION",)))
     specs.append(band_spec(eq_path("euler_fp32.cu"), 4, 2, 2, real="float"))
     for name, nv, k, s in (("euler_fp32.cu", 4, 2, 2), ("heat_fp32.cu", 1, 1, 1)):
         specs.append(standard_spec(eq_path(name), nv, k, s, 1024, 1024, real="float"))
