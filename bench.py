@@ -57,8 +57,9 @@ def parse():
                     help="legs of the headline: swept (band kernel), standard, phases (five phase kernels); "
                          "both = swept and standard, all = the three")
     ap.add_argument("--batch", type=int, default=512, help="problems of the batched workload (0: skip)")
-    ap.add_argument("--batch-legs", default="tuned,swept,standard,plain",
-                    help="runs of the batched workload: engine with swept / standard kernels, plain one at a time")
+    ap.add_argument("--batch-legs", default="tuned,swept,standard,band,plain",
+                    help="runs of the batched workload: engine tuned / with swept, standard or band kernels, "
+                         "plain one at a time")
     ap.add_argument("--serve", type=int, default=48, help="requests of the served stream (0: skip)")
     ap.add_argument("--rate", type=float, default=4.0, help="served-stream arrival rate per second")
     ap.add_argument("--large", type=int, default=1, help="large single instances (heat 8192^2, Euler 2048^2)")
@@ -206,7 +207,7 @@ def _size_bucket(p):
     return "small(<=256^2)" if cells <= 256 * 256 else ("medium(<=640^2)" if cells <= 640 * 640 else "large")
 
 
-def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "plain"), buckets=True):
+def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "band", "plain"), buckets=True):
     """Independent heat and Euler problems, 200 steps each, FP64: the batched
     engine (instance-table launches, graph-replayed rounds) against solving
     them one at a time with the plain standard kernel.
@@ -215,7 +216,8 @@ def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "plain"), bu
     picked each class's block size, method and workgroup shapes on a sample
     of its problems (40 steps each; the time it took is reported); the
     "swept" and "standard" engines force one method with the default
-    shapes.  ``buckets`` adds a per-size breakdown: each size bucket run
+    shapes, the "band" engine the band method where a class has it (Euler;
+    heat on the standard method).  ``buckets`` adds a per-size breakdown: each size bucket run
     alone on the tuned engine and one at a time.  ``legs`` narrows the study
     (a profile of one engine run uses it)."""
     from pysweep_amd import autotune, baseline, batch, hip, workloads
@@ -237,7 +239,7 @@ def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "plain"), bu
 
     def engine_run(ps, method, tuner_):
         for p in ps:
-            p.method = method
+            p.method = method(p) if callable(method) else method
         eng = batch.Engine(tuner=tuner_)
         for p in ps:
             eng.submit(p, arrival=0.0)
@@ -247,8 +249,11 @@ def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "plain"), bu
         hip.synchronize()
         return time.perf_counter() - t0, eng
 
+    def band_where_possible(p):
+        return "band" if p.plugin == "euler" else "standard"
+
     for label, method, tu in (("engine_tuned", "auto", tuner), ("engine_swept", "swept", None),
-                              ("engine_standard", "standard", None)):
+                              ("engine_standard", "standard", None), ("engine_band", band_where_possible, None)):
         if label.split("_")[1] not in legs:
             continue
         sec, eng = engine_run(probs, method, tu)

@@ -297,6 +297,9 @@ typedef struct ps_inst {
     int c;               /* round counter: 0 FirstPrism, 1..G UpPrism,        */
                          /* G+1 LastPrism, G+2.. tail stages                 */
     int active;
+    int brows, bands, bsegs; /* band-method problems (G = -1, stages on       */
+                         /* BandMulti): output rows per wavefront, bands     */
+                         /* across y, row segments; else unused              */
 } ps_inst;
 
 typedef struct ps_multi_plan {
@@ -310,6 +313,12 @@ typedef struct ps_multi_plan {
                                        standard tile, each; one workgroup each */
     int threads, sthreads;
     ps_function up, ybridge, xbridge, octahedron, down, standard, advance;
+    void *bprefix;      /* device int[ninst+1]: prefix sums of band         */
+                        /* wavefronts (bands x bsegs of each band-method    */
+                        /* problem), or null                                */
+    void *bowner;       /* device int[total_bwaves]                         */
+    int total_bwaves;   /* band work items: one wavefront (a workgroup) each */
+    ps_function band;   /* BandMulti of the class's band module, or null    */
 } ps_multi_plan;
 
 /* Fill the owner tables of a plan from its prefix sums on the work stream
@@ -317,8 +326,10 @@ typedef struct ps_multi_plan {
 int ps_multi_index(const ps_multi_plan *plan);
 
 /* Run `rounds` rounds: each round launches UpPyramidMulti, XBridgeMulti,
- * OctahedronMulti, DownPyramidMulti, YBridgeMulti, StandardMulti and
- * ps_advance; every problem executes the phases of its own cycle c. */
+ * OctahedronMulti, DownPyramidMulti, YBridgeMulti, StandardMulti, BandMulti
+ * and ps_advance; every problem executes the phases of its own cycle c (a
+ * band-method problem one whole step on the rounds whose stage begins one,
+ * nothing on the others). */
 int ps_multi_run(const ps_multi_plan *plan, int rounds);
 int ps_multi_run_timed(const ps_multi_plan *plan, int rounds, ps_run_stats *stats);
 

@@ -1,4 +1,4 @@
-"""Autotuner for instance-table classes: method (swept or standard), block
+"""Autotuner for instance-table classes: method (swept, standard or band), block
 size b, and the occupancy of each kernel family (workgroups per CU, set
 through the swept tile's LDS window and lanes, and the standard tile's
 shape and lanes).
@@ -35,10 +35,12 @@ DEFAULTS = {
 SHAPE_KEYS = ("slots", "threads", "stile", "sthreads")
 
 #: candidates per plugin: the swept block sizes, the swept LDS window (a
-#: smaller window fits more workgroups on a CU), and the standard kernel with
-#: tiles of 256 or 512 lanes (8 or 4 workgroups of LDS per CU for Euler)
+#: smaller window fits more workgroups on a CU), the standard kernel with
+#: tiles of 256 or 512 lanes (8 or 4 workgroups of LDS per CU for Euler), and
+#: for a plugin with a band sweep of one step per launch the band kernel (b
+#: only names the class there)
 CANDIDATES = {
-    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "slots": 3},
+    "euler": [{"b": 16}, {"b": 32}, {"b": 16, "slots": 3}, {"b": 16, "method": "band"},
               {"b": 16, "method": "standard"},
               {"b": 16, "method": "standard", "stile": [8, 16], "sthreads": 256},
               {"b": 16, "method": "standard", "stile": [8, 32], "sthreads": 512}],
@@ -74,6 +76,14 @@ def workgroups_per_cu(lds_bytes, regs, threads):
     return int(min(by_lds, by_regs))
 
 
+def _band_ok(plugin, precision, its):
+    """Whether the class can run the band method (kernels.band_multi_capable)."""
+    from pysweep_amd import batch, kernels
+    from pysweep_amd.equations import path as eq_path
+    py, cu64, cu32, nv, ops, _ = batch.PLUGINS[plugin]
+    return kernels.band_multi_capable(eq_path(cu64 if precision == "fp64" else cu32), nv, ops, its)
+
+
 def _key(plugin, precision, its):
     return f"{plugin}/{precision}/its{its}"
 
@@ -100,7 +110,8 @@ class Tuner:
         raise ValueError(f"grid {nx}x{ny} has no supported block size")
 
     def method(self, plugin, precision, its):
-        """"swept" or "standard" for problems submitted with method="auto"."""
+        """"swept", "standard" or "band" for problems submitted with
+        method="auto"."""
         from pysweep_amd import batch
         r = self.records.get(_key(plugin, precision, its), {})
         return r.get("method", batch.DEFAULT_METHOD.get((plugin, precision), "swept"))
@@ -163,13 +174,17 @@ class Tuner:
     @staticmethod
     def _occupancy(eng, plugin, precision, its, b, method):
         """Workgroups per CU of the class's kernel that carries the work:
-        the Octahedron (swept) or the standard kernel."""
+        the Octahedron (swept), the standard kernel or BandMulti (one
+        wavefront per workgroup)."""
         from pysweep_amd import hip
         cls = eng.classes.get((plugin, precision, its, b))
         if cls is None:
             return None
-        fn, threads = ((cls.fn["Octahedron"], cls.spec.threads) if method == "swept"
-                       else (cls.fn["Standard"], cls.sspec.threads))
+        if method == "band":
+            fn, threads = cls.band_multi(), cls.bspec.threads
+        else:
+            fn, threads = ((cls.fn["Octahedron"], cls.spec.threads) if method == "swept"
+                           else (cls.fn["Standard"], cls.sspec.threads))
         info = hip.function_info(fn)
         return workgroups_per_cu(info["lds_bytes"], info["regs"], threads)
 
@@ -197,6 +212,7 @@ class Tuner:
                 return probs
 
             cands = [c for c in CANDIDATES[plugin]
-                     if all(p.ic.shape[1] % c["b"] == 0 and p.ic.shape[2] % c["b"] == 0 for p in pick)]
+                     if all(p.ic.shape[1] % c["b"] == 0 and p.ic.shape[2] % c["b"] == 0 for p in pick)
+                     and (c.get("method") != "band" or _band_ok(plugin, precision, its))]
             out[_key(plugin, precision, its)] = self.tune(plugin, precision, its, make, candidates=cands, save=save)
         return out

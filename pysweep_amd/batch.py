@@ -7,7 +7,12 @@ and a table ``ps_inst[]`` on the device.  One *round* (``ps_multi_run``)
 launches every phase kernel once over all tiles of all problems in the
 table; each problem executes the phases of its own cycle (FirstPrism,
 UpPrism, LastPrism or an exact-step tail stage), so problems admitted at
-different times, with different grids and step counts, share launches.
+different times, with different grids and step counts, share launches.  A
+plugin with a band sweep of one step per launch (Euler) adds the band
+method: such a problem's rounds are its stages, and the class's BandMulti
+launch takes one whole step of every band-method problem whose stage
+begins one, each problem in row segments of its own length
+(``kernels.band_multi_rows``).
 
 ``Engine`` keeps the state of every admitted problem resident in HBM, admits
 queued problems while the HBM budget allows, retires finished ones (their
@@ -67,7 +72,8 @@ class Problem:
     b: int = None
     exact: bool = True
     tag: object = None
-    method: str = "swept"  # "swept"; "standard": every stage on the standard kernel; "auto": the faster one
+    method: str = "swept"  # "swept"; "standard": every stage on the standard kernel; "band": every step on the
+    #                        band kernel (plugins with a band sweep, kernels.band_multi_capable); "auto": the tuner's
 
 
 class _HostConsts:
@@ -144,6 +150,11 @@ def shipped_class_specs():
     for plugin, precision, its in (("euler", "fp64", 2), ("heat", "fp64", 1), ("heat", "fp64", 2)):
         for cand in autotune.CANDIDATES[plugin]:
             out.extend(class_specs(plugin, precision, its, cand["b"], **autotune.shape_of(cand)))
+    # the instance-table band modules of the classes that have one
+    for plugin, precision in (("euler", "fp64"), ("euler", "fp32")):
+        py, cu64, cu32, nv, ops, its = PLUGINS[plugin]
+        out.append(kernels.band_spec(eq_path(cu64 if precision == "fp64" else cu32), nv, ops, its,
+                                     real="double" if precision == "fp64" else "float", multi=True))
     seen, uniq = set(), []
     for sp in out:
         if sp not in seen:
@@ -176,6 +187,8 @@ class PluginClass:
         self._solo = None
         self._band = None
         self.cu = eq_path(cu64 if precision == "fp64" else cu32)
+        self.band_multi_ok = kernels.band_multi_capable(self.cu, nv, ops, its)
+        self.bspec = None  # the instance-table band module's spec, once loaded
 
     def band_module(self, nx, ny):
         """The class's band module, loaded on first use, when the band kernel
@@ -190,9 +203,23 @@ class PluginClass:
             self._band = (spec, m, m.get_function("Band"))
         return self._band
 
+    def band_multi(self):
+        """The class's BandMulti kernel (its instance-table band module,
+        loaded on first use)."""
+        if not self.band_multi_ok:
+            raise ValueError(f"class {self.key} has no instance-table band form (kernels.band_multi_capable)")
+        if "BandMulti" not in self.fn:
+            self.bspec = kernels.band_spec(self.cu, self.nv, self.ops, self.its, real=self.real, multi=True)
+            m = kernels.load(self.bspec)
+            self.modules.append(m)
+            self.fn["BandMulti"] = m.get_function("BandMulti")
+        return self.fn["BandMulti"]
+
     def levels(self, method):
         """Ring levels of a problem: the swept window 2P+s (core/solver.py:123),
-        or s+1 for a problem that runs every stage on the standard kernel."""
+        or s+1 for a problem that runs every stage on the standard kernel or
+        every step on the band kernel (which reads the step's first level and
+        writes its last)."""
         return 2 * self.P + self.its if method == "swept" else self.its + 1
 
     def ring_bytes(self, nx, ny, method="swept"):
@@ -414,11 +441,11 @@ class _ClassTable:
         self.cls = cls
         self.running = []
         self.dirty = True
-        self.insts = self.prefix = self.sprefix = None
+        self.insts = self.prefix = self.sprefix = self.bprefix = None
         self.cap = 0
         self.plan = None
         self.graph = None
-        self.owner = self.sowner = None
+        self.owner = self.sowner = self.bowner = None
         self.flags = None
 
     def needs(self, active):
@@ -444,19 +471,30 @@ class _ClassTable:
             self.insts = hip.DeviceArray(self.cap * ctypes.sizeof(hip.Inst))
             self.prefix = hip.DeviceArray((self.cap + 1) * 4)
             self.sprefix = hip.DeviceArray((self.cap + 1) * 4)
+            self.bprefix = hip.DeviceArray((self.cap + 1) * 4)
         isz, psz = n * ctypes.sizeof(hip.Inst), 4 * (n + 1)
-        host = stage(isz + 2 * psz) if stage else None
+        host = stage(isz + 3 * psz) if stage else None
         if host is not None:
             arr = (hip.Inst * n).from_address(host.ptr)
-            words = host.view(numpy.int32, ((isz + 2 * psz) // 4,))
-            pre, spre = words[isz // 4: isz // 4 + n + 1], words[isz // 4 + n + 1:]
+            words = host.view(numpy.int32, ((isz + 3 * psz) // 4,))
+            o = isz // 4
+            pre, spre, bpre = words[o:o + n + 1], words[o + n + 1:o + 2 * n + 2], words[o + 2 * n + 2:]
             pre[:] = 0
             spre[:] = 0
+            bpre[:] = 0
         else:
             arr = (hip.Inst * n)()
             pre = numpy.zeros(n + 1, dtype=numpy.int32)
             spre = numpy.zeros(n + 1, dtype=numpy.int32)
+            bpre = numpy.zeros(n + 1, dtype=numpy.int32)
         sbx, sby = cls.sspec.bx, cls.sspec.by
+        # band-method problems taking part: row segments of one launch
+        banded = [t for t in self.running if t.method == "band" and (active is None or t in active)]
+        geo, wout = {}, 1
+        if banded:
+            cls.band_multi()
+            geo = dict(zip(banded, kernels.band_multi_rows(cls.bspec, [(t.nx, t.ny) for t in banded])))
+            wout = kernels.band_out_columns(cls.bspec)
         for i, t in enumerate(self.running):
             st = arr[i]
             st.ring, st.consts = t.ring.ptr, t.consts
@@ -465,22 +503,30 @@ class _ClassTable:
             st.stiles_x, st.stiles_y = -(-t.nx // sbx), -(-t.ny // sby)
             st.G, st.tail, st.c = t.times.G, t.times.tail, t.c
             st.active = int(active is None or t in active)
+            band = t.method == "band"
+            rows = geo.get(t, 0) if band else 0
+            st.brows, st.bands = rows, (-(-t.ny // wout) if rows else 0)
+            st.bsegs = -(-t.nx // rows) if rows else 0
             # problems without swept cycles (G = -1) take no swept work items,
-            # problems sitting out of the rounds none at all
+            # problems sitting out of the rounds none at all; band-method
+            # problems take band work items instead of standard tiles
             run = bool(st.active)
             pre[i + 1] = pre[i] + (st.tiles_x * st.tiles_y if t.times.G >= 0 and run else 0)
-            spre[i + 1] = spre[i] + (st.stiles_x * st.stiles_y if t.times.tail and run else 0)
+            spre[i + 1] = spre[i] + (st.stiles_x * st.stiles_y if t.times.tail and run and not band else 0)
+            bpre[i + 1] = bpre[i] + (st.bands * st.bsegs if t.times.tail and run and band else 0)
         if host is not None:
             host.upload_async(self.insts.ptr, isz)
             host.upload_async(self.prefix.ptr, psz, isz)
             host.upload_async(self.sprefix.ptr, psz, isz + psz)
+            host.upload_async(self.bprefix.ptr, psz, isz + 2 * psz)
         else:
             hip.check(hip._htod(self.insts.ptr, ctypes.addressof(arr), ctypes.sizeof(arr)), "ps_memcpy_htod")
             self.prefix.upload(pre)
             self.sprefix.upload(spre)
+            self.bprefix.upload(bpre)
         self.flags = tuple(bool(arr[i].active) for i in range(n))
         # owner tables (problem of each work item), grown geometrically
-        for attr, total in (("owner", int(pre[-1])), ("sowner", int(spre[-1]))):
+        for attr, total in (("owner", int(pre[-1])), ("sowner", int(spre[-1])), ("bowner", int(bpre[-1]))):
             buf = getattr(self, attr, None)
             if total and (buf is None or buf.nbytes < 4 * total):
                 if buf is not None:
@@ -494,7 +540,10 @@ class _ClassTable:
                                   threads=cls.spec.threads, sthreads=cls.sspec.threads,
                                   up=f["UpPyramid"], ybridge=f["YBridge"], xbridge=f["XBridge"],
                                   octahedron=f["Octahedron"], down=f["DownPyramid"], standard=f["Standard"],
-                                  advance=f["advance"])
+                                  advance=f["advance"],
+                                  bprefix=self.bprefix.ptr,
+                                  bowner=self.bowner.ptr if getattr(self, "bowner", None) else None,
+                                  total_bwaves=int(bpre[-1]), band=f.get("BandMulti"))
         hip.multi_index(self.plan)
         self.graph = None
         self.dirty = False
@@ -744,6 +793,17 @@ class Engine:
                 r = min(r, max(self.poll_rounds, fit))
         return max(1, min(r, self.max_rounds_ahead))
 
+    @staticmethod
+    def _band_rounds(active, R):
+        """R rounded up to whole steps while a band-method problem takes part:
+        its BandMulti launch on the round that begins a step computes the
+        whole step and writes only its last level, so every host step leaves
+        such a problem at the start of a step (where the one-problem kernels
+        can take it over).  Other problems are unaffected by the extra
+        rounds: their kernels skip stages past a problem's last one."""
+        s = max((t.cls.its for t in active if t.method == "band"), default=1)
+        return -(-R // s) * s
+
     def _run_solo(self, t, R):
         """Rounds c .. c+R-1 of a problem that runs alone, with the plain
         one-problem kernels (the same phases and stages as the batched
@@ -827,7 +887,7 @@ class Engine:
         if self.solo and len(active) == 1:
             # one problem: the plain kernels, as a one-at-a-time solve runs it
             t = next(iter(active))
-            R = self._rounds_ahead(active)
+            R = self._band_rounds(active, self._rounds_ahead(active))
             self._run_solo(t, R)
             self.tables[t.cls.key].dirty = True  # its round counter moved on the host only
             self.rounds += R
@@ -838,7 +898,7 @@ class Engine:
                 self._retire(t)
             self._collect()
             return
-        R = self._rounds_ahead(active)
+        R = self._band_rounds(active, self._rounds_ahead(active))
         for key, tab in self.tables.items():
             if tab.dirty or tab.needs(active):
                 tab.rebuild(active, stage=self._stage)
@@ -936,8 +996,13 @@ class Engine:
         if nv != cls.nv or nx % b or ny % b:
             raise ValueError(f"problem shape {p.ic.shape} does not fit class {cls.key}")
         N = schedule.requested_steps(p.globals[0], p.globals[1], p.globals[2])
+        if method not in ("swept", "standard", "band"):
+            raise ValueError(f"unknown method {method!r} (swept, standard, band or auto)")
+        if method == "band" and not cls.band_multi_ok:
+            raise ValueError(f"method='band' needs a plugin band sweep of one step per launch; class {cls.key} "
+                             "has none")
         t.method = method
-        if method == "standard":
+        if method in ("standard", "band"):
             t.times = schedule.standard_times(N, b, cls.ops, cls.its)
         else:
             t.times = schedule.swept_times(N, b, cls.ops, cls.its, exact=p.exact)

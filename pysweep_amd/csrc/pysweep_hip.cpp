@@ -326,7 +326,7 @@ int level_sums(const void *level, int fp32, int nv, size_t cells, double *sums, 
 
 extern "C" {
 
-int ps_version(void) { return 4; }
+int ps_version(void) { return 5; }
 
 const char *ps_last_error(void) { return g_error.c_str(); }
 
@@ -848,6 +848,12 @@ static int multi_round(const ps_multi_plan *p)
         const unsigned g = static_cast<unsigned>(p->total_stiles);
         if ((rc = launch_kind(p->standard, g, static_cast<unsigned>(p->sthreads), sargs, PS_K_STANDARD))) return rc;
     }
+    if (p->total_bwaves > 0 && p->band) {
+        void *bprefix = p->bprefix, *bowner = p->bowner;
+        void *bargs[] = {&insts, &bprefix, &bowner, &n};
+        // one wavefront per workgroup: the workgroup's LDS holds its problem's constants
+        if ((rc = launch_kind(p->band, static_cast<unsigned>(p->total_bwaves), 64, bargs, PS_K_BAND))) return rc;
+    }
     void *aargs[] = {&insts, &n};
     return launch(p->advance, static_cast<unsigned>((n + 255) / 256), 256, aargs);
 }
@@ -887,6 +893,12 @@ int ps_multi_index(const ps_multi_plan *plan)
                            static_cast<const int *>(plan->sprefix), plan->ninst, static_cast<int *>(plan->sowner));
         PS_HIP(hipGetLastError());
     }
+    if (plan->total_bwaves > 0) {
+        if (!plan->bowner || !plan->bprefix) return fail("multi plan without band owner table");
+        hipLaunchKernelGGL(ps_owner_kernel, dim3(plan->ninst), dim3(256), 0, stream(),
+                           static_cast<const int *>(plan->bprefix), plan->ninst, static_cast<int *>(plan->bowner));
+        PS_HIP(hipGetLastError());
+    }
     return 0;
 }
 
@@ -894,8 +906,11 @@ int ps_multi_run(const ps_multi_plan *plan, int rounds)
 {
     PS_NEED_INIT();
     if (!plan || !plan->insts || !plan->prefix || !plan->advance) return fail("incomplete multi plan");
-    if ((plan->total_tiles > 0 && !plan->owner) || (plan->total_stiles > 0 && !plan->sowner))
+    if ((plan->total_tiles > 0 && !plan->owner) || (plan->total_stiles > 0 && !plan->sowner) ||
+        (plan->total_bwaves > 0 && !plan->bowner))
         return fail("multi plan without owner tables (ps_multi_index)");
+    if (plan->total_bwaves > 0 && (!plan->band || !plan->bprefix))
+        return fail("multi plan with band work items but no BandMulti kernel");
     if (plan->total_tiles > 0 && (!plan->up || !plan->ybridge || !plan->xbridge || !plan->octahedron || !plan->down))
         return fail("multi plan without phase kernels");
     for (int r = 0; r < rounds; ++r) {

@@ -683,10 +683,15 @@ __device__ __forceinline__ eu_band_row<R> eu_band_cell(const eu_band_win<R> &w, 
     return o;
 }
 
+#define EU_GLOBAL __attribute__((address_space(1)))
+
 template <class R>
 struct eu_band_ctx {
-    R *state;
-    const R *in;  /* level A */
+    /* the ring in the global address space: its accesses compile to global
+     * loads and stores also where the pointer was read from memory (an
+     * instance table), whose address space the compiler cannot infer */
+    EU_GLOBAL R *state;
+    const EU_GLOBAL R *in;  /* level A */
     size_t cells;
     int s0, T, nx, ny, wlo, hh, x0, rows, xs, col, oc, lane;
     bool out_col;
@@ -695,7 +700,7 @@ struct eu_band_ctx {
 template <class R>
 __device__ __forceinline__ void eu_band_load(const eu_band_ctx<R> &c, int row, eu_band_row<R> &o)
 {
-    const R *p = c.in + (size_t)row * c.ny + c.col;
+    const EU_GLOBAL R *p = c.in + (size_t)row * c.ny + c.col;
 #pragma unroll
     for (int v = 0; v < 4; ++v) o.v[v] = p[v * c.cells];
 }
@@ -705,7 +710,7 @@ __device__ __forceinline__ void eu_band_store(const eu_band_ctx<R> &c, int l, in
 {
     if (l >= c.wlo && c.out_col && row >= c.x0 && row < c.x0 + c.rows) {
         const int slot = c.s0 + l >= c.T ? c.s0 + l - c.T : c.s0 + l;
-        R *dst = c.state + (size_t)slot * NV * c.cells + (size_t)row * c.ny + c.oc;
+        EU_GLOBAL R *dst = c.state + (size_t)slot * NV * c.cells + (size_t)row * c.ny + c.oc;
 #pragma unroll
         for (int v = 0; v < 4; ++v) dst[v * c.cells] = o.v[v];
     }
@@ -754,22 +759,20 @@ __device__ __forceinline__ void eu_band_step(const eu_band_ctx<R> &c, int tau, e
 }
 
 template <class R>
-__device__ __forceinline__ void ps_band_sweep(R *state, int gts0, int A, const ps_band_grid &g)
+__device__ __forceinline__ void ps_band_sweep(R *state, int gts0, int A, const ps_band_grid &g, int band, int seg)
 {
     static_assert(PS_BAND_H == 2, "the Euler band sweep takes one RK2 step (two stages) per launch");
     constexpr int WOUT = EU_BAND_WOUT;
     const int lane = threadIdx.x & 63;
-    int band, seg;
-    if (!ps_band_wave(g, band, seg)) return;  /* whole wavefronts leave */
     (void)gts0;  /* launches start on full steps: level 1 is a half stage, level 2 a final one */
     eu_band_ctx<R> c;
-    c.state = state;
+    c.state = (EU_GLOBAL R *)state;  /* device memory */
     c.cells = (size_t)g.nx * g.ny;
     c.s0 = A % g.T;
     c.T = g.T;
     c.nx = g.nx;
     c.ny = g.ny;
-    c.in = state + (size_t)c.s0 * NV * c.cells;
+    c.in = c.state + (size_t)c.s0 * NV * c.cells;
     c.hh = g.hh;
     c.wlo = g.wlo;
     c.x0 = seg * g.rows;

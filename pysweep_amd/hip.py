@@ -54,7 +54,8 @@ class Inst(ctypes.Structure):
     """ps_inst: one problem of an instance-table launch."""
     _fields_ = [("ring", c_void_p), ("consts", c_void_p), ("nx", c_int), ("ny", c_int), ("T", c_int),
                 ("tiles_x", c_int), ("tiles_y", c_int), ("stiles_x", c_int), ("stiles_y", c_int),
-                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int)]
+                ("G", c_int), ("tail", c_int), ("c", c_int), ("active", c_int),
+                ("brows", c_int), ("bands", c_int), ("bsegs", c_int)]
 
 
 class MultiPlan(ctypes.Structure):
@@ -63,7 +64,8 @@ class MultiPlan(ctypes.Structure):
                 ("total_tiles", c_int), ("total_stiles", c_int),
                 ("threads", c_int), ("sthreads", c_int),
                 ("up", c_void_p), ("ybridge", c_void_p), ("xbridge", c_void_p), ("octahedron", c_void_p),
-                ("down", c_void_p), ("standard", c_void_p), ("advance", c_void_p)]
+                ("down", c_void_p), ("standard", c_void_p), ("advance", c_void_p),
+                ("bprefix", c_void_p), ("bowner", c_void_p), ("total_bwaves", c_int), ("band", c_void_p)]
 
 
 class RunStats(ctypes.Structure):

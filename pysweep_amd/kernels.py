@@ -77,7 +77,7 @@ class ModuleSpec:
                                          pfd=self.pfd, defines=self.defines, unroll=self.unroll)
         if self.kind == "band":
             return compiler.band_source(user, name, self.nv, self.ops, self.its, self.levels, real=self.real,
-                                        defines=self.defines)
+                                        defines=self.defines, multi=self.multi)
         if self.kind == "standard":
             return compiler.standard_source(user, name, self.nv, self.ops, self.its, self.bx, self.by, self.threads,
                                             mpss=self.mpss, real=self.real, multi=self.multi, generic=self.generic,
@@ -157,6 +157,13 @@ def band_capable(user_path, nv, ops, its):
     return has_band_sweep(user_path)
 
 
+def band_multi_capable(user_path, nv, ops, its):
+    """Whether problems of this plugin can run on the instance-table band
+    kernel (BandMulti): a plugin band sweep of one whole step per launch, so
+    that a problem's ring holds s + 1 levels, as on the standard kernel."""
+    return its == 2 and has_band_sweep(user_path) and not has_point_update(user_path)
+
+
 #: cells from which a plugin's own band sweep (Euler: one column per lane,
 #: a 2-column halo per level) is the faster swept kernel: its 64-column
 #: bands leave small grids too few wavefronts.  Measured on MI355X (Euler,
@@ -217,16 +224,11 @@ def band_rows(spec, nx, ny, simds=BAND_SIMDS):
     1833, 128 1970, 192 1868, 256 1856."""
     wout = band_out_columns(spec)
     bands = -(-ny // wout)
-    H, k = spec.levels, spec.ops
-    point = has_point_update(spec.user_path)
+    work_of = band_wave_work(spec)
     best = None
     for rows in range(min(4, nx), min(nx, 256) + 1):
         waves = bands * -(-nx // rows)
-        steps = 3 * -(-(rows + 2 * k * H) // 3)
-        if point:
-            work = sum(steps - 2 * l for l in range(1, H + 1)) + 2 * H
-        else:
-            work = sum(steps - (2 * k * l - 1) for l in range(1, H + 1)) + 4
+        work = work_of(rows)
         m = -(-waves // simds)
         cost = work / 0.815 if m == 1 else m * work
         if best is None or cost <= best[0]:
@@ -234,14 +236,76 @@ def band_rows(spec, nx, ny, simds=BAND_SIMDS):
     return best[1]
 
 
-def band_spec(user_path, nv, ops, its, real="double", levels=None, defines=()):
+def band_wave_work(spec):
+    """The work of one band wavefront as a function of its output rows, in
+    sweep steps (each computing every level of the launch for one row): the
+    segment plus the launch's x halo, rounded up to the step loop's
+    unrolling (3); level l computes from the step its first valid row appears
+    on; a few steps' worth of set-up is added."""
+    H, k = spec.levels, spec.ops
+    point = has_point_update(spec.user_path)
+
+    def work(rows):
+        steps = 3 * -(-(rows + 2 * k * H) // 3)
+        if point:
+            return sum(steps - 2 * l for l in range(1, H + 1)) + 2 * H
+        return sum(steps - (2 * k * l - 1) for l in range(1, H + 1)) + 4
+    return work
+
+
+#: row caps an instance-table band launch chooses from
+BAND_MULTI_CAPS = (4, 6, 8, 10, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 112, 128, 160, 192, 224, 256)
+
+
+def band_multi_rows(spec, shapes, simds=BAND_SIMDS):
+    """Output rows per wavefront of each problem (nx, ny) of an
+    instance-table band launch.  One cap R serves them all: a problem of nx
+    rows takes ceil(nx / R) segments of equal length.  R minimises the
+    busiest SIMD's time, as band_rows does for one problem: with W
+    wavefronts of at most w_max steps, ceil(W / SIMDs) * w_max (every SIMD
+    running its share of the longest ones), and at least the total work
+    spread over the SIMDs plus half the longest wavefront (the last ones to
+    start); one wavefront alone on its SIMD issues about 0.815 of the
+    time."""
+    import numpy
+    if not shapes:
+        return []
+    wout = band_out_columns(spec)
+    work_of = band_wave_work(spec)
+    nx = numpy.array([s[0] for s in shapes], dtype=numpy.int64)
+    bands = -(-numpy.array([s[1] for s in shapes], dtype=numpy.int64) // wout)
+    best = None
+    for cap in BAND_MULTI_CAPS:
+        segs = -(-nx // cap)
+        rows = -(-nx // segs)
+        work = numpy.array([work_of(int(r)) for r in rows])
+        waves = bands * segs
+        W, w_max, total = int(waves.sum()), int(work.max()), float((waves * work).sum())
+        m = -(-W // simds)
+        cost = w_max / 0.815 if m == 1 else max(m * w_max, total / simds + 0.5 * w_max)
+        if best is None or cost <= best[0]:
+            best = (cost, rows)
+        if cap >= int(nx.max()):
+            break
+    return [int(r) for r in best[1]]
+
+
+def band_spec(user_path, nv, ops, its, real="double", levels=None, defines=(), multi=False):
+    """The band module of a plugin; ``multi``: its instance-table form
+    (BandMulti, one wavefront per workgroup), for a plugin band sweep of one
+    step per launch."""
     if not band_capable(user_path, nv, ops, its):
         raise ValueError(f"{os.path.basename(user_path)} (nv={nv}, k={ops}, s={its}) has no band form: a point "
                          "update (PS_HAS_POINT, one variable, k=1) or a band sweep (PS_HAS_BAND), s <= 2")
     if levels is None and has_band_sweep(user_path) and not has_point_update(user_path):
         levels = its  # one step per launch: the registers hold a few rows of every level
-    spec = ModuleSpec("band", os.path.abspath(user_path), nv, ops, its, threads=256, real=real,
-                      levels=band_levels(its, levels, real), defines=tuple(defines))
+    if multi and not band_multi_capable(user_path, nv, ops, its):
+        raise ValueError(f"{os.path.basename(user_path)} has no instance-table band form: a band sweep "
+                         "(PS_HAS_BAND) of one whole step per launch (s = 2)")
+    if multi and levels is not None and levels != its:
+        raise ValueError(f"an instance-table band launch takes one step ({its} levels), not {levels}")
+    spec = ModuleSpec("band", os.path.abspath(user_path), nv, ops, its, threads=64 if multi else 256, real=real,
+                      multi=bool(multi), levels=band_levels(its, levels, real), defines=tuple(defines))
     if band_out_columns(spec) < 8:
         raise ValueError(f"band of {spec.levels} levels leaves fewer than 8 output columns")
     return spec

@@ -336,3 +336,94 @@ def test_oldest_first_policy_matches_oracle(device, capacity):
         steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
         want = oracle_stages(step, p.ic, p.its, p.its * steps)
         assert np.array_equal(t.result, want), (p.plugin, p.ic.shape, capacity)
+
+
+def _band_mix():
+    """Euler problems on the band method (grids narrower than one band of 56
+    output columns, ragged last bands, non-square, odd step counts), an
+    Euler problem of the same class on each of the other methods, and a heat
+    problem beside them."""
+    out = [_euler_problem(64, 96, 9), _euler_problem(96, 64, 5, eps=4.0), _euler_problem(128, 176, 7, eps=2.0),
+           _euler_problem(160, 128, 6, xc=-1.0), _euler_problem(64, 64, 4), _euler_problem(96, 96, 8, eps=1.5)]
+    for p, _ in out[:4]:
+        p.method = "band"
+    out[4][0].method = "standard"
+    out[5][0].method = "swept"
+    return out + [_heat_problem(64, 64, 30)]
+
+
+def _check_oracle(tickets, mix, tag=""):
+    for t, (p, step) in zip(tickets, mix):
+        steps = int(round((p.globals[1] - p.globals[0]) / p.globals[2]))
+        want = oracle_stages(step, p.ic, p.its, p.its * steps)
+        assert np.array_equal(t.result, want), (p.plugin, p.method, p.ic.shape, tag)
+
+
+@pytest.mark.parametrize("solo", [True, False])
+def test_band_method_batched_matches_oracle_bitwise(device, solo):
+    """Band-method problems share BandMulti launches with each other while
+    the swept, standard and heat problems take their own kernels in the same
+    rounds: every result is bitwise the oracle's, with the one-problem
+    kernels taking over the last problem running (solo=True) or not."""
+    mix = _band_mix()
+    tickets, eng = _run([p for p, _ in mix], solo=solo)
+    _check_oracle(tickets, mix, f"solo={solo}")
+    assert all(t.conservation["drift"] < 1e-13 for t in tickets)
+
+
+def test_band_method_hands_over_between_solo_and_batched_rounds(device):
+    """A band-method problem runs alone on the one-problem kernels, then in
+    batched rounds once others arrive, then alone again: each host step
+    leaves it at the start of a step (the batched launch writes only a
+    step's last level), and every result is bitwise the oracle's."""
+    from pysweep_amd.batch import Engine
+    mix = _band_mix()
+    first, rest = mix[2], [mix[0], mix[4], mix[6]]
+    eng = Engine(max_rounds_ahead=3)
+    t0 = eng.submit(first[0], arrival=0.0)
+    eng.step()
+    assert t0.c > 1 and (t0.c - 1) % 2 == 0, t0.c  # 3 rounds asked, a whole number of steps run
+    tickets = [eng.submit(p, arrival=0.0) for p, _ in rest]
+    while not eng.idle():
+        eng.step()
+        if not t0.done:
+            assert (t0.c - 1) % 2 == 0, t0.c
+    eng.run()
+    _check_oracle([t0] + tickets, [first] + rest)
+
+
+def test_band_method_fp32_alone_equals_in_mix_and_standard(device):
+    """FP32-state problems on the band method in a mix: each result equals
+    the same problem run alone on the standard method, bitwise."""
+    import dataclasses
+    mix = _band_mix()
+    f32 = [_euler_problem(96, 64, 7, eps=2.0, precision="fp32"), _euler_problem(64, 112, 5, precision="fp32")]
+    for p, _ in f32:
+        p.method = "band"
+    tickets, _ = _run([p for p, _ in mix + f32])
+    for i, (p, _) in enumerate(f32):
+        alone, _ = _run([dataclasses.replace(p, method="standard")], solo=False)
+        assert np.array_equal(alone[0].result, tickets[len(mix) + i].result), i
+
+
+@pytest.mark.parametrize("rounds", [1, 3])
+def test_band_method_graph_replay_equals_eager(device, rounds):
+    """Band-method rounds replayed from captured hipGraphs of 1 or 3 rounds
+    give the same bits as eager rounds."""
+    mix = _band_mix()
+    a, _ = _run([p for p, _ in mix], use_graphs=True, graph_rounds=rounds, solo=False)
+    b, _ = _run([p for p, _ in mix], use_graphs=False, solo=False)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.result, y.result)
+
+
+def test_band_method_rejected_without_a_band_sweep(device):
+    """Heat has a point update but no band sweep of one step per launch:
+    method="band" is refused at admission with a clear error."""
+    from pysweep_amd.batch import Engine
+    p, _ = _heat_problem(64, 64, 4)
+    p.method = "band"
+    eng = Engine()
+    eng.submit(p, arrival=0.0)
+    with pytest.raises(ValueError, match="band sweep"):
+        eng.run()

@@ -557,3 +557,63 @@ def test_band_forms_and_launch_shapes():
         kernels.band_spec(euler, 4, 2, 2, levels=16)  # 64 - 2*2*16 < 8 output columns
     src = h.source()
     assert src.index("ps_band_grid") < src.index("ps_point") < src.index("Band(PS_REAL")
+
+
+def test_abi_structs_match_the_header(tmp_path):
+    """Every ctypes mirror of a C-ABI struct has the size and field offsets
+    of the declaration in include/pysweep_hip.h (compiled with gcc)."""
+    import shutil
+    import subprocess
+    from pysweep_amd import hip
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not found")
+    mirrors = {"ps_grid": hip.Grid, "ps_swept_plan": hip.SweptPlan, "ps_standard_plan": hip.StandardPlan,
+               "ps_band_plan": hip.BandPlan, "ps_inst": hip.Inst, "ps_multi_plan": hip.MultiPlan,
+               "ps_run_stats": hip.RunStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pysweep_hip.h"', "int main(void) {"]
+    for cname, cls in mirrors.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, value = line.split()
+        got[(cname, field)] = int(value)
+    for cname, cls in mirrors.items():
+        assert got[(cname, "size")] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[(cname, fname)] == getattr(cls, fname).offset, (cname, fname)
+
+
+def test_band_multi_rows_and_whole_step_rounds():
+    """The instance-table band launch: each problem's row segments cover its
+    grid in equal parts; a batch with more wavefronts takes longer segments;
+    only a plugin band sweep of one step per launch has the form; host steps
+    run whole steps while a band-method problem takes part."""
+    from types import SimpleNamespace
+    from pysweep_amd import batch
+    spec = kernels.band_spec(eq_path("euler.cu"), 4, 2, 2, multi=True)
+    assert spec.multi and spec.threads == 64 and spec.levels == 2
+    shapes = [(2048, 2048), (64, 96), (100, 56), (1, 8)]
+    for (nx, ny), r in zip(shapes, kernels.band_multi_rows(spec, shapes)):
+        segs = -(-nx // r)
+        assert 1 <= r <= nx and (segs - 1) * r < nx <= segs * r, (nx, r)
+    assert kernels.band_multi_rows(spec, []) == []
+    few = kernels.band_multi_rows(spec, [(1024, 1024)])[0]
+    many = kernels.band_multi_rows(spec, [(1024, 1024)] * 16)[0]
+    assert many > few
+    with pytest.raises(ValueError, match="instance-table band"):
+        kernels.band_spec(eq_path("heat.cu"), 1, 1, 2, multi=True)
+    assert not kernels.band_multi_capable(eq_path("heat.cu"), 1, 1, 2)
+    assert kernels.band_multi_capable(eq_path("euler_fp32.cu"), 4, 2, 2)
+    band = SimpleNamespace(method="band", cls=SimpleNamespace(its=2))
+    std = SimpleNamespace(method="standard", cls=SimpleNamespace(its=2))
+    assert batch.Engine._band_rounds([std], 3) == 3
+    assert batch.Engine._band_rounds([std, band], 3) == 4
+    assert batch.Engine._band_rounds([band], 6) == 6
+    assert any(sp.kind == "band" and sp.multi for sp in batch.shipped_class_specs())
