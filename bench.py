@@ -240,6 +240,7 @@ def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "band", "pla
     def engine_run(ps, method, tuner_):
         for p in ps:
             p.method = method(p) if callable(method) else method
+        batch.release_rings()  # each engine allocates its own rings (its peak HBM is its own)
         eng = batch.Engine(tuner=tuner_)
         for p in ps:
             eng.submit(p, arrival=0.0)
@@ -533,6 +534,7 @@ def served_workload(n, rate, seed=1):
            "cell_steps": cells, "engine_policy": SERVE_POLICY}
 
     def serve(**kw):
+        batch.release_rings()  # the engine's peak HBM is its own
         eng = batch.Engine(hbm_budget=96 << 30, **kw)
         hip.synchronize()
         eng.t0 = eng.clock()

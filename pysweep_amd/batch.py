@@ -49,11 +49,14 @@ PLUGINS = {
 
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
-#: side (Euler FP64 1024^2: standard 12.4 GCUPS, swept b=16 9.5; heat 8192^2:
-#: standard 299, swept b=32 183, b=64 287).
+#: side.  Euler: the band method (512-problem mix of 64..1024 sides, tuner
+#: sample: band 19.95 GCUPS, standard 17.1, swept b=32 12.6; the saturated
+#: served stream of 128..2048 sides under the "all" policy: 29.8 solves/s
+#: against 23.8, its latencies unchanged).  Heat: standard (8192^2: standard
+#: 299, swept b=32 183, b=64 287; heat has no band method of one step).
 DEFAULT_METHOD = {
-    ("euler", "fp64"): "standard",
-    ("euler", "fp32"): "standard",
+    ("euler", "fp64"): "band",
+    ("euler", "fp32"): "band",
     ("heat", "fp64"): "standard",
     ("heat", "fp32"): "standard",
 }
@@ -365,6 +368,23 @@ def _copy_pool():
     return _COPY_POOL[0]
 
 
+def _copy_into(dst, src, piece=4 << 20):
+    """dst[...] = src (with the assignment's casting), in slabs of rows on
+    the copy threads when it is large (numpy drops the GIL while copying):
+    one thread copies about 25 GB/s into page-locked memory on MI355X hosts,
+    which left the GPU waiting while a batch's initial conditions went in."""
+    n = src.shape[1]
+    parts = min(8, n, src.nbytes // piece)
+    if parts < 2:
+        dst[...] = src
+        return
+    edges = [n * k // parts for k in range(parts + 1)]
+
+    def one(k):
+        dst[:, edges[k]:edges[k + 1]] = src[:, edges[k]:edges[k + 1]]
+    list(_copy_pool().map(one, range(parts)))
+
+
 _FREE_RINGS = []  # staging rings of finished engines, for the next engine
 STAGING_BYTES = 1 << 30
 
@@ -377,6 +397,31 @@ def _take_ring(nbytes=0):
             _FREE_RINGS.remove(r)
             return r
     return _StagingRing(nbytes)
+
+
+_DEVICE_RINGS = {}  # context handle -> free problem rings of the engines on that context
+
+
+def _ring_pool():
+    """The free problem rings of the calling thread's context, shared by its
+    engines: a finished engine's rings serve the next engine instead of
+    being freed (hipFree waits for the whole device, so an engine collected
+    while another runs would stall it once per ring).  A ring given back
+    after its last copy was queued is safe to reuse for any later work of
+    the same context, which runs on the same stream."""
+    key = hip.current_context()
+    pool = _DEVICE_RINGS.get(key)
+    if pool is None:
+        pool = _DEVICE_RINGS[key] = _Pool(hip.DeviceArray)
+    return pool
+
+
+def release_rings():
+    """Free the pooled problem rings of the calling thread's context (waits
+    for the device); engines made later allocate afresh."""
+    pool = _DEVICE_RINGS.pop(hip.current_context(), None)
+    if pool is not None:
+        pool.trim(0)
 
 
 #: bytes after a problem's ring: its constant block (256) and the FP64 sums
@@ -663,7 +708,9 @@ class Engine:
         self.finished = []
         self.in_use = 0  # device bytes of the running problems' rings
         self.peak_in_use = 0  # peak of in_use plus the pooled free rings
-        self.rings = _Pool(hip.DeviceArray)
+        # the context's free rings, within this engine's budget
+        self.rings = _ring_pool()
+        self.rings.trim(self.hbm_budget)
         # page-locked staging, made (and touched) here rather than in the
         # first timed rounds; given back when the engine falls idle
         self._ring = _take_ring()
@@ -1022,7 +1069,7 @@ class Engine:
         # levels 0 .. its-1 hold the initial condition: one upload from a
         # staging buffer, then device copies
         host = self._stage(lvl + 256)
-        host.view(cls.dtype, (nv, nx, ny))[...] = p.ic
+        _copy_into(host.view(cls.dtype, (nv, nx, ny)), p.ic)
         host.view(numpy.uint8, (lvl + 256,))[lvl:lvl + len(hc.buf)] = numpy.frombuffer(bytes(hc.buf), numpy.uint8)
         host.upload_async(t.ring.ptr, lvl)
         host.upload_async(t.consts, len(hc.buf), lvl)

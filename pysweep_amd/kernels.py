@@ -274,11 +274,13 @@ def band_multi_rows(spec, shapes, simds=BAND_SIMDS):
     work_of = band_wave_work(spec)
     nx = numpy.array([s[0] for s in shapes], dtype=numpy.int64)
     bands = -(-numpy.array([s[1] for s in shapes], dtype=numpy.int64) // wout)
+    top = BAND_MULTI_CAPS[-1]
+    table = numpy.array([0] + [work_of(r) for r in range(1, top + 1)], dtype=numpy.int64)
     best = None
     for cap in BAND_MULTI_CAPS:
         segs = -(-nx // cap)
         rows = -(-nx // segs)
-        work = numpy.array([work_of(int(r)) for r in rows])
+        work = table[rows]
         waves = bands * segs
         W, w_max, total = int(waves.sum()), int(work.max()), float((waves * work).sum())
         m = -(-W // simds)

@@ -142,8 +142,9 @@ def test_reused_ring_after_async_retirement(device):
     ring its predecessor just left, while that ring's final level may still
     be on its way to the host.  Stream order must keep every result intact,
     and the pool must serve every problem from the first ring."""
-    from pysweep_amd.batch import Engine
+    from pysweep_amd.batch import Engine, release_rings
     mix = [_heat_problem(64, 64, 20 + 3 * i, kx=1 + i % 3, amp=0.5 + 0.25 * i) for i in range(5)]
+    release_rings()  # no free rings left by earlier engines
     eng = Engine(max_active=1)
     tickets = [eng.submit(p, arrival=0.0) for p, _ in mix]
     eng.run()
@@ -241,8 +242,10 @@ def test_fp32_state_error_against_fp64_oracle(device):
 
 
 def test_auto_method_and_round_batching_match_oracle(device):
-    """method="auto" (the faster kernel per class) and engines that run one
-    round per host step or many give the oracle's results bitwise."""
+    """method="auto" (the faster kernel per class: Euler takes the band
+    method) and engines that run one round per host step (two while a
+    band-method problem takes part) or many give the oracle's results
+    bitwise."""
     mix = _mix()
     for p, _ in mix:
         p.method = "auto"
@@ -252,7 +255,7 @@ def test_auto_method_and_round_batching_match_oracle(device):
         steps = int((p.globals[1] - p.globals[0]) / p.globals[2])
         want = oracle_stages(step, p.ic, p.its, p.its * steps)
         assert np.array_equal(t.result, want) and np.array_equal(u.result, want)
-    assert {t.method for t in a} <= {"swept", "standard"}
+    assert {t.method for t in a} <= {"swept", "standard", "band"}
 
 
 def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(device, tmp_path):
@@ -427,3 +430,24 @@ def test_band_method_rejected_without_a_band_sweep(device):
     eng.submit(p, arrival=0.0)
     with pytest.raises(ValueError, match="band sweep"):
         eng.run()
+
+
+def test_engines_share_free_rings(device):
+    """A finished engine leaves its rings to the next engine of the same
+    context (no device free between them), within the new engine's HBM
+    budget; release_rings frees them."""
+    from pysweep_amd.batch import Engine, release_rings
+    release_rings()
+    mix = _mix()
+    a, ea = _run([p for p, _ in mix])
+    held = {id(r) for r in ea.rings.free}
+    assert ea.rings.bytes > 0 and ea.in_use == 0
+    b, eb = _run([p for p, _ in mix])
+    assert eb.rings is ea.rings and {id(r) for r in eb.rings.free} == held  # every ring reused, none new
+    for x, y in zip(a, b):
+        assert np.array_equal(x.result, y.result)
+    total = ea.rings.bytes
+    small = Engine(hbm_budget=total // 2)
+    assert 0 < total and small.rings.bytes <= total // 2
+    release_rings()
+    assert Engine().rings.bytes == 0
