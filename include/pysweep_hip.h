@@ -67,6 +67,9 @@ int ps_context_pop(void);
 /* The context current on the calling thread, NULL when it uses the process
  * context. */
 int ps_context_current(ps_context *ctx);
+/* Device of the calling thread's current context (a second context for
+ * its copies is made on the same one). */
+int ps_context_device(int *device);
 
 /* Compile HIP source text for gfx950 with hiprtc; no device is needed.
  * On success *code (size *size) is a code object the caller frees with
@@ -143,6 +146,13 @@ int ps_event_synchronize(ps_event event);
  * never blocks. */
 int ps_event_query(ps_event event, int *done);
 int ps_event_destroy(ps_event event);
+/* The current context's work stream waits, on the device, for the work
+ * recorded before `event` on any stream of the same device (another
+ * context's); the host does not wait.  Lets a second context's stream
+ * carry copies beside the work stream's kernels, each kernel that needs a
+ * copy waiting for that copy alone (no counterpart in the reference, whose
+ * copies and launches share one stream). */
+int ps_stream_wait_event(ps_event event);
 
 /* ---------------------------------------------------------------------- */
 /* Native phase sequencing.                                                */

@@ -412,8 +412,37 @@ def _ring_pool():
     key = hip.current_context()
     pool = _DEVICE_RINGS.get(key)
     if pool is None:
+        if _forget_context not in hip.CONTEXT_CLOSE_HOOKS:
+            hip.CONTEXT_CLOSE_HOOKS.append(_forget_context)
         pool = _DEVICE_RINGS[key] = _Pool(hip.DeviceArray)
     return pool
+
+
+_COPY_CONTEXTS = {}  # context handle -> the context whose stream carries its engines' copies
+
+
+def _copy_context():
+    """The context whose stream carries the copies of the calling thread's
+    context's engines: one per work context, so that every copy into or out
+    of a pooled ring is ordered on one stream (a download out of a retired
+    problem's ring before the next problem's upload into it)."""
+    key = hip.current_context()
+    ctx = _COPY_CONTEXTS.get(key)
+    if ctx is None:
+        if _forget_context not in hip.CONTEXT_CLOSE_HOOKS:
+            hip.CONTEXT_CLOSE_HOOKS.append(_forget_context)
+        ctx = _COPY_CONTEXTS[key] = hip.Context(hip.current_device())
+    return ctx
+
+
+def _forget_context(handle):
+    """A closed work context's pooled rings and copy context go with it."""
+    pool = _DEVICE_RINGS.pop(handle, None)
+    if pool is not None:
+        pool.trim(0)
+    ctx = _COPY_CONTEXTS.pop(handle, None)
+    if ctx is not None:
+        ctx.close()
 
 
 def release_rings():
@@ -662,6 +691,13 @@ class Engine:
     steps, the stages before the first full step on the standard kernel;
     results are bitwise unchanged.
 
+    ``copy_stream``: initial conditions go up and results come down on the
+    stream of a second context (one per work context, shared by its
+    engines), beside the rounds: the work stream waits on the device for
+    each upload before using the problem, each download for the problem's
+    last round.  False queues them on the work stream, in order with the
+    rounds.
+
     ``tuner``: the ``autotune.Tuner`` whose records give each class its
     block size, the method of problems submitted with method="auto" and the
     workgroup shapes of its kernels; "default" loads the stored records
@@ -671,7 +707,7 @@ class Engine:
     def __init__(self, hbm_budget=None, max_active=None, use_graphs=True, tuner="default", clock=None,
                  poll_rounds=4, max_rounds_ahead=1 << 20, policy="all", max_inflight=2, solo=True,
                  capacity=1 << 20, graph_rounds=8, admit_bytes=256 << 20, admit_rounds=4, poison=False,
-                 band_solo=True):
+                 band_solo=True, copy_stream=True):
         hip.ensure_init(0)
         self.policy = policy
         self.admit_bytes = admit_bytes  # initial-condition bytes staged per step at most
@@ -711,6 +747,9 @@ class Engine:
         # the context's free rings, within this engine's budget
         self.rings = _ring_pool()
         self.rings.trim(self.hbm_budget)
+        # uploads and downloads on the stream of the context's copy context,
+        # beside the rounds (or, copy_stream=False, on the work stream)
+        self._copy = _copy_context() if copy_stream else None
         # page-locked staging, made (and touched) here rather than in the
         # first timed rounds; given back when the engine falls idle
         self._ring = _take_ring()
@@ -1062,8 +1101,6 @@ class Engine:
         # one buffer: the ring, then the constant block (lvl is a multiple
         # of 256 bytes: nx and ny are multiples of b >= 8)
         t.ring = self.rings.take(need, limit=self.hbm_budget - self.in_use)
-        if self.poison:  # every level NaN until written (stream ordered, before the uploads)
-            t.ring.fill_bytes(0xFF, T * lvl)
         t.consts = t.ring.ptr + T * lvl
         t.sums = t.consts + 256
         # levels 0 .. its-1 hold the initial condition: one upload from a
@@ -1071,8 +1108,23 @@ class Engine:
         host = self._stage(lvl + 256)
         _copy_into(host.view(cls.dtype, (nv, nx, ny)), p.ic)
         host.view(numpy.uint8, (lvl + 256,))[lvl:lvl + len(hc.buf)] = numpy.frombuffer(bytes(hc.buf), numpy.uint8)
-        host.upload_async(t.ring.ptr, lvl)
-        host.upload_async(t.consts, len(hc.buf), lvl)
+        if self._copy is not None:
+            self._copy.push()
+        try:
+            if self.poison:  # every level NaN until written (stream ordered, before the uploads)
+                t.ring.fill_bytes(0xFF, T * lvl)
+            host.upload_async(t.ring.ptr, lvl)
+            host.upload_async(t.consts, len(hc.buf), lvl)
+            if self._copy is not None:
+                up = hip.Event()
+                up.record()
+        finally:
+            if self._copy is not None:
+                self._copy.pop()
+        if self._copy is not None:
+            # everything the work stream does with the problem (and every
+            # fence recorded there later) follows its upload
+            up.wait()
         for A in range(1, cls.its):
             hip.dtod_async(t.ring.ptr + (A % T) * lvl, t.ring.ptr, lvl)
         hip.level_sums_async(t.ring.ptr, cls.rb == 4, nv, nx * ny, t.sums)
@@ -1096,7 +1148,9 @@ class Engine:
     def _retire(self, t):
         """Queue the copy of the final level to the host (the ticket is
         finished when it has arrived) and return the ring to the pool: later
-        work on the stream that reuses it runs after the copy."""
+        work that reuses it runs after the copy (on the work stream, or on
+        the copy stream, where the next problem's upload into it follows
+        the copy)."""
         cls = t.cls
         nv = cls.nv
         T = t.T
@@ -1105,9 +1159,19 @@ class Engine:
         fin = t.ring.ptr + (t.final_level % T) * lvl
         hip.level_sums_async(fin, cls.rb == 4, nv, t.nx * t.ny, t.sums + 128)
         piece = self._stage(lvl + 256)  # the level, then its sums (lvl is a multiple of 256)
-        piece.download_async(fin, lvl)
-        piece.download_async(t.sums, 256, lvl)
-        self._fence(t, (piece, cls.dtype, (nv, t.nx, t.ny)))
+        if self._copy is None:
+            piece.download_async(fin, lvl)
+            piece.download_async(t.sums, 256, lvl)
+            self._fence(t, (piece, cls.dtype, (nv, t.nx, t.ny)))
+        else:
+            # on the copy stream, after the problem's last round and sums
+            last = hip.Event()
+            last.record()
+            with self._copy:
+                last.wait()
+                piece.download_async(fin, lvl)
+                piece.download_async(t.sums, 256, lvl)
+                self._fence(t, (piece, cls.dtype, (nv, t.nx, t.ny)))  # its event on the copy stream
         self.rings.give(t.ring)
         t.ring, t.consts, t.sums = None, 0, 0
         self.in_use -= t.nbytes
@@ -1115,7 +1179,6 @@ class Engine:
         tab = self.tables[cls.key]
         tab.running.remove(t)
         tab.dirty = True
-        self.finished.append(t)
 
 
 def percentile(xs, q):

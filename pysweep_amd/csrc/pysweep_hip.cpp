@@ -326,7 +326,7 @@ int level_sums(const void *level, int fp32, int nv, size_t cells, double *sums, 
 
 extern "C" {
 
-int ps_version(void) { return 5; }
+int ps_version(void) { return 6; }
 
 const char *ps_last_error(void) { return g_error.c_str(); }
 
@@ -411,6 +411,14 @@ int ps_context_current(ps_context *ctx)
 {
     if (!ctx) return fail("ps_context_current: null output");
     *ctx = t_stack.empty() ? nullptr : t_stack.back();
+    return 0;
+}
+
+int ps_context_device(int *device)
+{
+    PS_NEED_INIT();
+    if (!device) return fail("ps_context_device: null output");
+    *device = current()->device;
     return 0;
 }
 
@@ -655,6 +663,14 @@ int ps_event_query(ps_event event, int *done)
 int ps_event_destroy(ps_event event)
 {
     if (event) PS_HIP(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return 0;
+}
+
+int ps_stream_wait_event(ps_event event)
+{
+    PS_NEED_INIT();
+    if (!event) return fail("ps_stream_wait_event: null event");
+    PS_HIP(hipStreamWaitEvent(stream(), static_cast<hipEvent_t>(event), 0));
     return 0;
 }
 

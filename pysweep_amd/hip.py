@@ -109,6 +109,7 @@ _ctx_destroy = _sig("ps_context_destroy", c_void_p)
 _ctx_push = _sig("ps_context_push", c_void_p)
 _ctx_pop = _sig("ps_context_pop")
 _ctx_current = _sig("ps_context_current", ctypes.POINTER(c_void_p))
+_ctx_device = _sig("ps_context_device", ctypes.POINTER(c_int))
 _compile = _sig("ps_compile", c_char_p, c_char_p, ctypes.POINTER(c_char_p), c_int,
                 ctypes.POINTER(c_void_p), ctypes.POINTER(c_size_t), ctypes.POINTER(c_void_p))
 _free_host = _sig("ps_free_host", c_void_p, restype=None)
@@ -137,6 +138,7 @@ _event_create = _sig("ps_event_create", ctypes.POINTER(c_void_p))
 _event_record = _sig("ps_event_record", c_void_p)
 _event_elapsed = _sig("ps_event_elapsed_ms", c_void_p, c_void_p, ctypes.POINTER(ctypes.c_float))
 _event_destroy = _sig("ps_event_destroy", c_void_p)
+_stream_wait = _sig("ps_stream_wait_event", c_void_p)
 _event_sync = _sig("ps_event_synchronize", c_void_p)
 _event_query = _sig("ps_event_query", c_void_p, ctypes.POINTER(c_int))
 _swept_run = _sig("ps_swept_run", ctypes.POINTER(SweptPlan), c_int, c_int, c_int)
@@ -160,13 +162,14 @@ _graph_destroy = _sig("ps_graph_destroy", c_void_p)
 
 #: every symbol the header declares (checked by the CPU test suite)
 EXPORTS = ("ps_version", "ps_last_error", "ps_device_count", "ps_init", "ps_shutdown", "ps_context_create",
-           "ps_context_destroy", "ps_context_push", "ps_context_pop", "ps_context_current", "ps_compile", "ps_compiler_version",
+           "ps_context_destroy", "ps_context_push", "ps_context_pop", "ps_context_current", "ps_context_device",
+           "ps_compile", "ps_compiler_version",
            "ps_free_host", "ps_module_load", "ps_module_unload", "ps_module_get_global",
            "ps_module_get_function", "ps_function_info", "ps_malloc", "ps_free", "ps_memcpy_htod",
            "ps_memcpy_dtoh", "ps_memcpy_dtod", "ps_memset", "ps_host_malloc", "ps_host_free",
            "ps_memcpy_htod_async", "ps_memcpy_dtoh_async", "ps_memcpy_dtod_async", "ps_launch", "ps_synchronize",
            "ps_event_create", "ps_event_record", "ps_event_elapsed_ms", "ps_event_synchronize", "ps_event_query",
-           "ps_event_destroy",
+           "ps_event_destroy", "ps_stream_wait_event",
            "ps_swept_run", "ps_standard_run", "ps_swept_run_timed", "ps_standard_run_timed",
            "ps_band_run", "ps_band_run_timed", "ps_copy_f64",
            "ps_level_sums", "ps_level_sums_noalloc",
@@ -219,6 +222,13 @@ def current_context():
     h = c_void_p()
     check(_ctx_current(ctypes.byref(h)), "ps_context_current")
     return h.value
+
+
+def current_device():
+    """Device of the calling thread's current context."""
+    d = c_int()
+    check(_ctx_device(ctypes.byref(d)), "ps_context_device")
+    return d.value
 
 
 class Context:
@@ -533,6 +543,12 @@ class Event:
         done = c_int()
         check(_event_query(self.h, ctypes.byref(done)), "ps_event_query")
         return bool(done.value)
+
+    def wait(self):
+        """The current context's work stream waits (on the device) for the
+        work recorded before this event, on whichever stream it was
+        recorded; the host does not wait."""
+        check(_stream_wait(self.h), "ps_stream_wait_event")
 
     def elapsed_ms(self, stop):
         ms = ctypes.c_float()

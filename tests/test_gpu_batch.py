@@ -137,34 +137,39 @@ def test_continuous_admission_under_hbm_budget(device):
         assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps))
 
 
-def test_reused_ring_after_async_retirement(device):
+@pytest.mark.parametrize("copy_stream", [True, False])
+def test_reused_ring_after_async_retirement(device, copy_stream):
     """One problem at a time (max_active=1): each admitted problem takes the
     ring its predecessor just left, while that ring's final level may still
-    be on its way to the host.  Stream order must keep every result intact,
-    and the pool must serve every problem from the first ring."""
+    be on its way to the host.  Stream order (the copy stream's, or the work
+    stream's) must keep every result intact, and the pool must serve every
+    problem from the first ring."""
     from pysweep_amd.batch import Engine, release_rings
     mix = [_heat_problem(64, 64, 20 + 3 * i, kx=1 + i % 3, amp=0.5 + 0.25 * i) for i in range(5)]
     release_rings()  # no free rings left by earlier engines
-    eng = Engine(max_active=1)
+    eng = Engine(max_active=1, copy_stream=copy_stream)
     tickets = [eng.submit(p, arrival=0.0) for p, _ in mix]
-    eng.run()
+    assert eng.run() is eng.finished
     assert [t.done for t in tickets] == [True] * 5
+    assert sorted(map(id, eng.finished)) == sorted(map(id, tickets))  # each ticket finished once
     assert len(eng.rings.free) == 1 and not eng.running and eng.in_use == 0
     for t, (p, step) in zip(tickets, mix):
         steps = int(round((p.globals[1] - p.globals[0]) / p.globals[2]))
         assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps)), t.problem.globals
 
 
-def test_small_staging_ring_wraps_waits_and_grows(device, monkeypatch):
+@pytest.mark.parametrize("copy_stream", [True, False])
+def test_small_staging_ring_wraps_waits_and_grows(device, monkeypatch, copy_stream):
     """A staging ring far smaller than the traffic: admissions and
-    retirements wait for earlier copies to free room, pieces wrap around
-    the ring, and a level larger than the ring makes a bigger one; every
-    result must still equal the oracle."""
+    retirements wait for earlier copies to free room (events of the work
+    and copy streams, released in order), pieces wrap around the ring, and a
+    level larger than the ring makes a bigger one; every result must still
+    equal the oracle."""
     from pysweep_amd import batch
     monkeypatch.setattr(batch, "STAGING_BYTES", 1 << 16)  # 64 KiB
     monkeypatch.setattr(batch, "_FREE_RINGS", [])
     mix = _mix()  # levels of 32 KiB .. 512 KiB
-    eng = batch.Engine(admit_bytes=1 << 16)
+    eng = batch.Engine(admit_bytes=1 << 16, copy_stream=copy_stream)
     assert eng._ring.size == 1 << 16
     tickets = [eng.submit(p, arrival=0.0) for p, _ in mix]
     eng.run()
@@ -451,3 +456,22 @@ def test_engines_share_free_rings(device):
     assert 0 < total and small.rings.bytes <= total // 2
     release_rings()
     assert Engine().rings.bytes == 0
+
+
+def test_engine_on_a_context_of_its_own(device):
+    """An engine on a thread's own context: its copies go through a copy
+    context made on that context's device, results equal the process
+    context's engine bitwise, and closing the context releases its pooled
+    rings and its copy context."""
+    from pysweep_amd import batch, hip
+    mix = _mix()
+    want, _ = _run([p for p, _ in mix])
+    ctx = hip.Context(hip.current_device())
+    with ctx:
+        got, eng = _run([p for p, _ in mix])
+        key = hip.current_context()
+        assert key in batch._COPY_CONTEXTS and key in batch._DEVICE_RINGS
+    for x, y in zip(want, got):
+        assert np.array_equal(x.result, y.result)
+    ctx.close()
+    assert key not in batch._COPY_CONTEXTS and key not in batch._DEVICE_RINGS

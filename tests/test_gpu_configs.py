@@ -12,13 +12,18 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def test_batched_mix_of_512_problems_equals_plain_solves(device):
+@pytest.mark.parametrize("method", ["swept", "auto"])
+def test_batched_mix_of_512_problems_equals_plain_solves(device, method):
     """Half heat, half Euler, sides 64..1024 (multiples of 32, non-square
-    allowed), 200 steps, FP64, through the default (tuner-driven) engine:
-    all 512 results equal the one-at-a-time plain solves bitwise; conserved
-    sums move by rounding only; HBM in use stays within the budget."""
+    allowed), 200 steps, FP64, through the default (tuner-driven) engine, on
+    the swept method or "auto" (Euler on the band method, heat on the
+    standard one): all 512 results equal the one-at-a-time plain solves
+    bitwise; conserved sums move by rounding only; HBM in use stays within
+    the budget."""
     from pysweep_amd import baseline, batch, workloads
     probs = workloads.batched_mix(n=512, steps=200, seed=0)
+    for p in probs:
+        p.method = method
     eng = batch.Engine()
     tickets = [eng.submit(p, arrival=0.0) for p in probs]
     eng.run()
@@ -27,6 +32,8 @@ def test_batched_mix_of_512_problems_equals_plain_solves(device):
     for t in tickets:
         assert t.conservation["drift"] < 1e-13, (t.problem.tag, t.conservation)
         assert t.division_exact
+    if method == "auto":
+        assert {t.method for t in tickets if t.problem.plugin == "euler"} == {"band"}
     for t, p in zip(tickets, probs):
         assert np.array_equal(t.result, baseline.solve_plain(p)), p.tag
 
