@@ -262,11 +262,14 @@ def band_multi_rows(spec, shapes, simds=BAND_SIMDS):
     instance-table band launch.  One cap R serves them all: a problem of nx
     rows takes ceil(nx / R) segments of equal length.  R minimises the
     busiest SIMD's time, as band_rows does for one problem: with W
-    wavefronts of at most w_max steps, ceil(W / SIMDs) * w_max (every SIMD
-    running its share of the longest ones), and at least the total work
-    spread over the SIMDs plus half the longest wavefront (the last ones to
-    start); one wavefront alone on its SIMD issues about 0.815 of the
-    time."""
+    wavefronts of at most w_max steps and m = ceil(W / SIMDs) per SIMD:
+    with m < 4, m * w_max (every SIMD running its share of the longest
+    ones, band_rows' measure); from 4 on, where wavefronts of different
+    lengths even out, the total work spread over the SIMDs plus one longest
+    wavefront (the last ones to start); one wavefront alone on its SIMD
+    issues about 0.815 of the time.  Measured on MI355X (the 256 Euler problems of
+    the 512-problem mix, engine seconds by cap): 24 0.940, 32 0.916, 48
+    0.894, 64 0.866, 96 0.849 (chosen), 128 0.855, 192 0.861, 256 0.908."""
     import numpy
     if not shapes:
         return []
@@ -284,7 +287,12 @@ def band_multi_rows(spec, shapes, simds=BAND_SIMDS):
         waves = bands * segs
         W, w_max, total = int(waves.sum()), int(work.max()), float((waves * work).sum())
         m = -(-W // simds)
-        cost = w_max / 0.815 if m == 1 else max(m * w_max, total / simds + 0.5 * w_max)
+        if m == 1:
+            cost = w_max / 0.815
+        elif m < 4:
+            cost = m * w_max
+        else:
+            cost = total / simds + w_max
         if best is None or cost <= best[0]:
             best = (cost, rows)
         if cap >= int(nx.max()):
