@@ -352,9 +352,10 @@ def _band_mix():
     Euler problem of the same class on each of the other methods, and a heat
     problem beside them."""
     out = [_euler_problem(64, 96, 9), _euler_problem(96, 64, 5, eps=4.0), _euler_problem(128, 176, 7, eps=2.0),
-           _euler_problem(160, 128, 6, xc=-1.0), _euler_problem(64, 64, 4), _euler_problem(96, 96, 8, eps=1.5)]
-    for p, _ in out[:4]:
-        p.method = "band"
+           _euler_problem(160, 128, 6, xc=-1.0), _euler_problem(64, 64, 4), _euler_problem(96, 96, 8, eps=1.5),
+           _euler_problem(48, 16, 5, eps=2.5), _euler_problem(16, 48, 3, yc=1.0)]
+    for i in (0, 1, 2, 3, 6, 7):  # 16 and 48 columns: narrower than a band and than the wavefront
+        out[i][0].method = "band"
     out[4][0].method = "standard"
     out[5][0].method = "swept"
     return out + [_heat_problem(64, 64, 30)]
@@ -386,7 +387,7 @@ def test_band_method_hands_over_between_solo_and_batched_rounds(device):
     step's last level), and every result is bitwise the oracle's."""
     from pysweep_amd.batch import Engine
     mix = _band_mix()
-    first, rest = mix[2], [mix[0], mix[4], mix[6]]
+    first, rest = mix[2], [mix[0], mix[4], mix[8]]
     eng = Engine(max_rounds_ahead=3)
     t0 = eng.submit(first[0], arrival=0.0)
     eng.step()
