@@ -521,6 +521,9 @@ class _ClassTable:
         self.graph = None
         self.owner = self.sowner = self.bowner = None
         self.flags = None
+        # arrays replaced by larger ones: freed once the engine is idle
+        # (hipFree waits for the whole device; queued rounds may still read them)
+        self.old = []
 
     def needs(self, active):
         """True when the table's run flags differ from ``active``."""
@@ -542,6 +545,7 @@ class _ClassTable:
             return
         if n > self.cap:
             self.cap = max(8, 2 * n)
+            self.old.extend(a for a in (self.insts, self.prefix, self.sprefix, self.bprefix) if a is not None)
             self.insts = hip.DeviceArray(self.cap * ctypes.sizeof(hip.Inst))
             self.prefix = hip.DeviceArray((self.cap + 1) * 4)
             self.sprefix = hip.DeviceArray((self.cap + 1) * 4)
@@ -604,7 +608,7 @@ class _ClassTable:
             buf = getattr(self, attr, None)
             if total and (buf is None or buf.nbytes < 4 * total):
                 if buf is not None:
-                    buf.close()
+                    self.old.append(buf)
                 setattr(self, attr, hip.DeviceArray(8 * total))
         f = cls.fn
         self.plan = hip.MultiPlan(insts=self.insts.ptr, prefix=self.prefix.ptr, sprefix=self.sprefix.ptr,
@@ -783,12 +787,21 @@ class Engine:
             r += 1
             if max_rounds and r >= max_rounds:
                 break
+        if self.idle():
+            self._release_old()
         if self.idle() and self._ring is not None:
             _FREE_RINGS.append(self._ring)
             self._ring = None
         if self.idle():
             self._check_division_range()
         return self.finished
+
+    def _release_old(self):
+        """Free the tables' replaced arrays (the engine is idle: no queued
+        work reads them)."""
+        for tab in self.tables.values():
+            while tab.old:
+                tab.old.pop().close()
 
     def _check_division_range(self):
         """Classes whose modules met a state outside the plugin's division
@@ -964,6 +977,7 @@ class Engine:
         if not self.running:
             self._fence()
             self._collect(wait=True)
+            self._release_old()  # every queued round finished before the last download
             if self.queue:
                 time.sleep(max(0.0, min(t.arrival for t in self.queue) - self.now()))
             return

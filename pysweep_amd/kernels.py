@@ -21,11 +21,45 @@ LDS_MAX = 160 * 1024
 _INCLUDE_RE = re.compile(r'^[ \t]*#[ \t]*include[ \t]+"([^"]+)"[ \t]*$', re.M)
 
 
+_SOURCES = {}  # absolute path -> ((file, mtime) of every file read, expanded text)
+
+
+def _mtime(path):
+    try:
+        return os.stat(path).st_mtime_ns
+    except OSError:
+        return None
+
+
+_MACROS = {}  # (absolute path, macro) -> (source text it was found in, answer)
+
+
+def _defines_macro(path, name):
+    """Whether the plugin source (with its includes) defines macro ``name``
+    on a line of its own; kept with the source text it was read from."""
+    text = read_plugin_source(path)
+    key = (os.path.abspath(path), name)
+    hit = _MACROS.get(key)
+    if hit is None or hit[0] is not text:
+        hit = _MACROS[key] = (text, re.search(r"^[ \t]*#[ \t]*define[ \t]+" + name + r"\b", text, re.M) is not None)
+    return hit[1]
+
+
 def read_plugin_source(path, _seen=None):
     """Plugin device source with ``#include "file"`` of sibling files
-    expanded in place (so the code-object cache key covers them)."""
+    expanded in place (so the code-object cache key covers them).  Kept
+    while none of the files it was read from has changed (the launch-shape
+    helpers ask for it on every table change of the batched engine)."""
     path = os.path.abspath(path)
-    seen = set() if _seen is None else _seen
+    if _seen is None:
+        hit = _SOURCES.get(path)
+        if hit is not None and all(_mtime(f) == m for f, m in hit[0]):
+            return hit[1]
+        seen = set()
+        text = read_plugin_source(path, seen)
+        _SOURCES[path] = (tuple((f, _mtime(f)) for f in seen), text)
+        return text
+    seen = _seen
     if path in seen:
         return ""
     seen.add(path)
@@ -111,14 +145,14 @@ class ModuleSpec:
 
 def has_level_update(user_path):
     """True when the plugin provides the cooperative level update."""
-    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_LEVEL\b", read_plugin_source(user_path), re.M) is not None
+    return _defines_macro(user_path, "PS_HAS_LEVEL")
 
 
 def has_point_update(user_path):
     """True when the plugin provides the point update of the band kernel
     (PS_HAS_POINT: a single-variable radius-1 stencil whose cell update
     takes its neighbours' values)."""
-    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_POINT\b", read_plugin_source(user_path), re.M) is not None
+    return _defines_macro(user_path, "PS_HAS_POINT")
 
 
 def band_levels(its, levels=None, real="double"):
@@ -145,7 +179,7 @@ def has_band_sweep(user_path):
     """True when the plugin provides a band sweep of its own (PS_HAS_BAND,
     the Euler plugin), compiled when the band kernel's declarations are
     present."""
-    return re.search(r"^[ \t]*#[ \t]*define[ \t]+PS_HAS_BAND\b", read_plugin_source(user_path), re.M) is not None
+    return _defines_macro(user_path, "PS_HAS_BAND")
 
 
 def band_capable(user_path, nv, ops, its):

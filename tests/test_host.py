@@ -617,3 +617,20 @@ def test_band_multi_rows_and_whole_step_rounds():
     assert batch.Engine._band_rounds([std, band], 3) == 4
     assert batch.Engine._band_rounds([band], 6) == 6
     assert any(sp.kind == "band" and sp.multi for sp in batch.shipped_class_specs())
+
+
+def test_plugin_source_cache_follows_edits(tmp_path):
+    """A plugin source (and its includes) is read again once a file it was
+    read from has changed, and so are the macro tests on it."""
+    inc = tmp_path / "part.cu"
+    top = tmp_path / "plug.cu"
+    inc.write_text("// nothing yet\n")
+    top.write_text('#include "part.cu"\n__device__ void step(double *s, int i, int g) {}\n')
+    assert not kernels.has_band_sweep(str(top))
+    first = kernels.read_plugin_source(str(top))
+    assert kernels.read_plugin_source(str(top)) is first  # unchanged files: the kept text
+    st = os.stat(inc)
+    inc.write_text("#define PS_HAS_BAND 1\n")
+    os.utime(inc, ns=(st.st_atime_ns, st.st_mtime_ns + 10 ** 9))
+    assert "PS_HAS_BAND" in kernels.read_plugin_source(str(top))
+    assert kernels.has_band_sweep(str(top))
