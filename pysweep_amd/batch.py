@@ -49,11 +49,13 @@ PLUGINS = {
 
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
-#: side.  Euler: the band method (512-problem mix of 64..1024 sides, tuner
-#: sample: band 19.95 GCUPS, standard 17.1, swept b=32 12.6; the saturated
-#: served stream of 128..2048 sides under the "all" policy: 29.8 solves/s
-#: against 23.8, its latencies unchanged).  Heat: standard (8192^2: standard
-#: 299, swept b=32 183, b=64 287; heat has no band method of one step).
+#: side.  Euler: the band method (512-problem mix of 64..1024 sides, the
+#: tuner's 24-problem 40-step sample: band 9.6 GCUPS, standard 8.4, swept
+#: b=32 6.4; the saturated served stream of 128..2048 sides under the "all"
+#: policy: 29.8 solves/s against 23.8, its latencies unchanged).  Heat:
+#: standard (8192^2: standard 299, swept b=32 183, b=64 287; heat has no band
+#: method of one step).  "auto" falls back to standard for a class without
+#: the band method (Engine._method_for).
 DEFAULT_METHOD = {
     ("euler", "fp64"): "band",
     ("euler", "fp32"): "band",
@@ -76,7 +78,8 @@ class Problem:
     exact: bool = True
     tag: object = None
     method: str = "swept"  # "swept"; "standard": every stage on the standard kernel; "band": every step on the
-    #                        band kernel (plugins with a band sweep, kernels.band_multi_capable); "auto": the tuner's
+    #                        band kernel (plugins with a band sweep, kernels.band_multi_capable); "auto": the
+    #                        tuner's record for the class, else DEFAULT_METHOD
 
 
 class _HostConsts:
@@ -399,17 +402,18 @@ def _take_ring(nbytes=0):
     return _StagingRing(nbytes)
 
 
-_DEVICE_RINGS = {}  # context handle -> free problem rings of the engines on that context
+_DEVICE_RINGS = {}  # (context handle, copies on the copy stream) -> free problem rings
 
 
-def _ring_pool():
+def _ring_pool(copy_stream=True):
     """The free problem rings of the calling thread's context, shared by its
-    engines: a finished engine's rings serve the next engine instead of
-    being freed (hipFree waits for the whole device, so an engine collected
-    while another runs would stall it once per ring).  A ring given back
-    after its last copy was queued is safe to reuse for any later work of
-    the same context, which runs on the same stream."""
-    key = hip.current_context()
+    engines that queue their copies the same way: a finished engine's rings
+    serve the next engine instead of being freed (hipFree waits for the
+    whole device, so an engine collected while another runs would stall it
+    once per ring).  A ring given back after its last copy was queued is
+    safe to reuse for later copies on the same stream (the copy stream, or
+    the work stream), so engines that differ there keep separate rings."""
+    key = (hip.current_context(), bool(copy_stream))
     pool = _DEVICE_RINGS.get(key)
     if pool is None:
         if _forget_context not in hip.CONTEXT_CLOSE_HOOKS:
@@ -437,9 +441,10 @@ def _copy_context():
 
 def _forget_context(handle):
     """A closed work context's pooled rings and copy context go with it."""
-    pool = _DEVICE_RINGS.pop(handle, None)
-    if pool is not None:
-        pool.trim(0)
+    for copy_stream in (True, False):
+        pool = _DEVICE_RINGS.pop((handle, copy_stream), None)
+        if pool is not None:
+            pool.trim(0)
     ctx = _COPY_CONTEXTS.pop(handle, None)
     if ctx is not None:
         ctx.close()
@@ -448,9 +453,10 @@ def _forget_context(handle):
 def release_rings():
     """Free the pooled problem rings of the calling thread's context (waits
     for the device); engines made later allocate afresh."""
-    pool = _DEVICE_RINGS.pop(hip.current_context(), None)
-    if pool is not None:
-        pool.trim(0)
+    for copy_stream in (True, False):
+        pool = _DEVICE_RINGS.pop((hip.current_context(), copy_stream), None)
+        if pool is not None:
+            pool.trim(0)
 
 
 #: bytes after a problem's ring: its constant block (256) and the FP64 sums
@@ -752,7 +758,7 @@ class Engine:
         self.in_use = 0  # device bytes of the running problems' rings
         self.peak_in_use = 0  # peak of in_use plus the pooled free rings
         # the context's free rings, within this engine's budget
-        self.rings = _ring_pool()
+        self.rings = _ring_pool(copy_stream)
         self.rings.trim(self.hbm_budget)
         # uploads and downloads on the stream of the context's copy context,
         # beside the rounds (or, copy_stream=False, on the work stream)
@@ -1075,7 +1081,7 @@ class Engine:
             b = self._block_size(p)
             cls = self._class_for(p, b)
             nv, nx, ny = p.ic.shape
-            method = self._method_for(p)
+            method = self._method_for(p, cls)
             need = cls.ring_bytes(nx, ny, method) + _TAIL_BYTES
             if self.in_use + need > self.hbm_budget:
                 if not self.running and need > self.hbm_budget:
@@ -1085,13 +1091,15 @@ class Engine:
             self._start(t, cls, b, need, method)
             staged += p.ic.size * cls.rb
 
-    def _method_for(self, p):
+    def _method_for(self, p, cls):
         """The kernel family a problem runs on ("auto": the tuner's record,
-        else the default table)."""
+        else the default table; the band method only where the class has
+        it, else the standard one)."""
         if p.method != "auto":
             return p.method
-        return (self.tuner.method(p.plugin, p.precision, p.its) if self.tuner
-                else DEFAULT_METHOD.get((p.plugin, p.precision), "swept"))
+        m = (self.tuner.method(p.plugin, p.precision, p.its) if self.tuner
+             else DEFAULT_METHOD.get((p.plugin, p.precision), "swept"))
+        return "standard" if m == "band" and not cls.band_multi_ok else m
 
     def _start(self, t, cls, b, need, method):
         p = t.problem

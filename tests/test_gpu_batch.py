@@ -455,6 +455,7 @@ def test_engines_share_free_rings(device):
     total = ea.rings.bytes
     small = Engine(hbm_budget=total // 2)
     assert 0 < total and small.rings.bytes <= total // 2
+    assert Engine(copy_stream=False).rings is not ea.rings  # copies queued elsewhere: rings of their own
     release_rings()
     assert Engine().rings.bytes == 0
 
@@ -471,8 +472,8 @@ def test_engine_on_a_context_of_its_own(device):
     with ctx:
         got, eng = _run([p for p, _ in mix])
         key = hip.current_context()
-        assert key in batch._COPY_CONTEXTS and key in batch._DEVICE_RINGS
+        assert key in batch._COPY_CONTEXTS and (key, True) in batch._DEVICE_RINGS
     for x, y in zip(want, got):
         assert np.array_equal(x.result, y.result)
     ctx.close()
-    assert key not in batch._COPY_CONTEXTS and key not in batch._DEVICE_RINGS
+    assert key not in batch._COPY_CONTEXTS and (key, True) not in batch._DEVICE_RINGS

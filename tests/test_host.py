@@ -634,3 +634,19 @@ def test_plugin_source_cache_follows_edits(tmp_path):
     os.utime(inc, ns=(st.st_atime_ns, st.st_mtime_ns + 10 ** 9))
     assert "PS_HAS_BAND" in kernels.read_plugin_source(str(top))
     assert kernels.has_band_sweep(str(top))
+
+
+def test_auto_method_falls_back_where_the_class_has_no_band_form():
+    """method="auto" resolves to the default table's method, and to the
+    standard one where that is "band" but the class cannot run it."""
+    from types import SimpleNamespace
+    from pysweep_amd import batch
+    eng = SimpleNamespace(tuner=None)
+    p = SimpleNamespace(method="auto", plugin="euler", precision="fp64", its=2)
+    assert batch.DEFAULT_METHOD[("euler", "fp64")] == "band"
+    assert batch.Engine._method_for(eng, p, SimpleNamespace(band_multi_ok=True)) == "band"
+    assert batch.Engine._method_for(eng, p, SimpleNamespace(band_multi_ok=False)) == "standard"
+    heat = SimpleNamespace(method="auto", plugin="heat", precision="fp64", its=1)
+    assert batch.Engine._method_for(eng, heat, SimpleNamespace(band_multi_ok=False)) == "standard"
+    fixed = SimpleNamespace(method="swept", plugin="euler", precision="fp64", its=2)
+    assert batch.Engine._method_for(eng, fixed, SimpleNamespace(band_multi_ok=True)) == "swept"
