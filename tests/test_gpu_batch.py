@@ -477,3 +477,19 @@ def test_engine_on_a_context_of_its_own(device):
         assert np.array_equal(x.result, y.result)
     ctx.close()
     assert key not in batch._COPY_CONTEXTS and (key, True) not in batch._DEVICE_RINGS
+
+
+@pytest.mark.parametrize("policy,capacity", [("short_first", 1 << 13), ("short_first", 1 << 15),
+                                             ("oldest_first", 1 << 14)])
+def test_band_method_under_selection_policies(device, policy, capacity):
+    """Round selection that leaves problems out of some rounds and takes
+    them back later (least remaining work first, or arrival order, up to a
+    small cell capacity), with staggered arrivals: band-method problems sit
+    out and resume at the start of a step, and every result is bitwise the
+    oracle's."""
+    from pysweep_amd.batch import Engine
+    mix = _band_mix()
+    eng = Engine(policy=policy, capacity=capacity, poll_rounds=3)
+    tickets = [eng.submit(p, arrival=0.003 * i) for i, (p, _) in enumerate(mix)]
+    eng.run()
+    _check_oracle(tickets, mix, f"{policy} {capacity}")
