@@ -216,8 +216,7 @@ def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "band", "pla
     picked each class's block size, method and workgroup shapes on a sample
     of its problems (40 steps each; the time it took is reported); the
     "swept" and "standard" engines force one method with the default
-    shapes, the "band" engine the band method where a class has it (Euler;
-    heat on the standard method).  ``buckets`` adds a per-size breakdown: each size bucket run
+    shapes, the "band" engine the band method (both plugins).  ``buckets`` adds a per-size breakdown: each size bucket run
     alone on the tuned engine and one at a time.  ``legs`` narrows the study
     (a profile of one engine run uses it)."""
     from pysweep_amd import autotune, baseline, batch, hip, workloads
@@ -250,11 +249,8 @@ def batched_workload(n, seed=0, legs=("tuned", "swept", "standard", "band", "pla
         hip.synchronize()
         return time.perf_counter() - t0, eng
 
-    def band_where_possible(p):
-        return "band" if p.plugin == "euler" else "standard"
-
     for label, method, tu in (("engine_tuned", "auto", tuner), ("engine_swept", "swept", None),
-                              ("engine_standard", "standard", None), ("engine_band", band_where_possible, None)):
+                              ("engine_standard", "standard", None), ("engine_band", "band", None)):
         if label.split("_")[1] not in legs:
             continue
         sec, eng = engine_run(probs, method, tu)

@@ -37,14 +37,13 @@ SHAPE_KEYS = ("slots", "threads", "stile", "sthreads")
 #: candidates per plugin: the swept block sizes, the swept LDS window (a
 #: smaller window fits more workgroups on a CU), the standard kernel with
 #: tiles of 256 or 512 lanes (8 or 4 workgroups of LDS per CU for Euler), and
-#: for a plugin with a band sweep of one step per launch the band kernel (b
-#: only names the class there)
+#: the band kernel (b only names the class there)
 CANDIDATES = {
     "euler": [{"b": 16}, {"b": 32}, {"b": 16, "slots": 3}, {"b": 16, "method": "band"},
               {"b": 16, "method": "standard"},
               {"b": 16, "method": "standard", "stile": [8, 16], "sthreads": 256},
               {"b": 16, "method": "standard", "stile": [8, 32], "sthreads": 512}],
-    "heat": [{"b": 16}, {"b": 32}, {"b": 64}, {"b": 32, "slots": 2},
+    "heat": [{"b": 16}, {"b": 32}, {"b": 64}, {"b": 32, "slots": 2}, {"b": 32, "method": "band"},
              {"b": 32, "method": "standard"},
              {"b": 32, "method": "standard", "stile": [32, 64], "sthreads": 256},
              {"b": 32, "method": "standard", "stile": [16, 128], "sthreads": 512}],

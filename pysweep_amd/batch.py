@@ -10,9 +10,9 @@ UpPrism, LastPrism or an exact-step tail stage), so problems admitted at
 different times, with different grids and step counts, share launches.  A
 plugin with a band sweep of one step per launch (Euler) adds the band
 method: such a problem's rounds are its stages, and the class's BandMulti
-launch takes one whole step of every band-method problem whose stage
-begins one, each problem in row segments of its own length
-(``kernels.band_multi_rows``).
+launch takes one whole launch's stages (one RK2 step for Euler, up to 12
+levels for heat) of every band-method problem whose stage begins one, each
+problem in row segments of its own length (``kernels.band_multi_rows``).
 
 ``Engine`` keeps the state of every admitted problem resident in HBM, admits
 queued problems while the HBM budget allows, retires finished ones (their
@@ -49,18 +49,19 @@ PLUGINS = {
 
 #: method of a Problem with method="auto" when no tuner record says
 #: otherwise: the faster one measured on MI355X for grids of 64..2048 cells a
-#: side.  Euler: the band method (512-problem mix of 64..1024 sides, the
+#: side: the band method.  Euler: 512-problem mix of 64..1024 sides, the
 #: tuner's 24-problem 40-step sample: band 9.6 GCUPS, standard 8.4, swept
 #: b=32 6.4; the saturated served stream of 128..2048 sides under the "all"
-#: policy: 29.8 solves/s against 23.8, its latencies unchanged).  Heat:
-#: standard (8192^2: standard 299, swept b=32 183, b=64 287; heat has no band
-#: method of one step).  "auto" falls back to standard for a class without
-#: the band method (Engine._method_for).
+#: policy: 29.8 solves/s against 23.8, its latencies unchanged.  Heat: the
+#: mix's 256 heat problems alone 0.107 s against 0.157 s on the standard
+#: kernel, for rings of 13 levels instead of 2 (the whole mix's peak HBM 14.4
+#: GB against 8.4).  "auto" falls back to standard for a class without the
+#: band method (Engine._method_for).
 DEFAULT_METHOD = {
     ("euler", "fp64"): "band",
     ("euler", "fp32"): "band",
-    ("heat", "fp64"): "standard",
-    ("heat", "fp32"): "standard",
+    ("heat", "fp64"): "band",
+    ("heat", "fp32"): "band",
 }
 
 
@@ -77,9 +78,9 @@ class Problem:
     b: int = None
     exact: bool = True
     tag: object = None
-    method: str = "swept"  # "swept"; "standard": every stage on the standard kernel; "band": every step on the
-    #                        band kernel (plugins with a band sweep, kernels.band_multi_capable); "auto": the
-    #                        tuner's record for the class, else DEFAULT_METHOD
+    method: str = "swept"  # "swept"; "standard": every stage on the standard kernel; "band": every stage on the
+    #                        band kernel (kernels.band_multi_capable); "auto": the tuner's record for the class,
+    #                        else DEFAULT_METHOD
 
 
 class _HostConsts:
@@ -157,8 +158,9 @@ def shipped_class_specs():
         for cand in autotune.CANDIDATES[plugin]:
             out.extend(class_specs(plugin, precision, its, cand["b"], **autotune.shape_of(cand)))
     # the instance-table band modules of the classes that have one
-    for plugin, precision in (("euler", "fp64"), ("euler", "fp32")):
-        py, cu64, cu32, nv, ops, its = PLUGINS[plugin]
+    for plugin, precision, its in (("euler", "fp64", 2), ("euler", "fp32", 2), ("heat", "fp64", 1),
+                                   ("heat", "fp64", 2), ("heat", "fp32", 1)):
+        py, cu64, cu32, nv, ops, _ = PLUGINS[plugin]
         out.append(kernels.band_spec(eq_path(cu64 if precision == "fp64" else cu32), nv, ops, its,
                                      real="double" if precision == "fp64" else "float", multi=True))
     seen, uniq = set(), []
@@ -194,6 +196,9 @@ class PluginClass:
         self._band = None
         self.cu = eq_path(cu64 if precision == "fp64" else cu32)
         self.band_multi_ok = kernels.band_multi_capable(self.cu, nv, ops, its)
+        # levels per band-method launch (a band-method problem's rounds come
+        # in whole launches of this many stages)
+        self.band_h = kernels.band_multi_levels(self.cu, its, self.real) if self.band_multi_ok else 0
         self.bspec = None  # the instance-table band module's spec, once loaded
 
     def band_module(self, nx, ny):
@@ -223,10 +228,12 @@ class PluginClass:
 
     def levels(self, method):
         """Ring levels of a problem: the swept window 2P+s (core/solver.py:123),
-        or s+1 for a problem that runs every stage on the standard kernel or
-        every step on the band kernel (which reads the step's first level and
-        writes its last)."""
-        return 2 * self.P + self.its if method == "swept" else self.its + 1
+        s+1 for a problem that runs every stage on the standard kernel, and
+        band_h+1 on the band method (a launch reads its first level and
+        writes its last, band_h later: s+1 for Euler's one-step launches)."""
+        if method == "swept":
+            return 2 * self.P + self.its
+        return self.band_h + 1 if method == "band" else self.its + 1
 
     def ring_bytes(self, nx, ny, method="swept"):
         return self.levels(method) * self.nv * nx * ny * self.rb
@@ -903,13 +910,17 @@ class Engine:
 
     @staticmethod
     def _band_rounds(active, R):
-        """R rounded up to whole steps while a band-method problem takes part:
-        its BandMulti launch on the round that begins a step computes the
-        whole step and writes only its last level, so every host step leaves
-        such a problem at the start of a step (where the one-problem kernels
-        can take it over).  Other problems are unaffected by the extra
-        rounds: their kernels skip stages past a problem's last one."""
-        s = max((t.cls.its for t in active if t.method == "band"), default=1)
+        """R rounded up to whole launches while a band-method problem takes
+        part: its BandMulti launch on the round that begins one computes
+        band_h stages and writes only the last, so every host step leaves
+        such a problem at the start of a launch (where the one-problem
+        kernels can take it over).  Other problems are unaffected by the
+        extra rounds: their kernels skip stages past a problem's last one,
+        and the one-problem path stops at it."""
+        s = 1
+        for t in active:
+            if t.method == "band":
+                s = math.lcm(s, t.cls.band_h)
         return -(-R // s) * s
 
     def _run_solo(self, t, R):
@@ -930,7 +941,7 @@ class Engine:
             for m in mods:
                 m.const_owner = t
         G, P, T = t.times.G, cls.P, t.T
-        c0, c1 = t.c, t.c + R
+        c0, c1 = t.c, min(t.c + R, t.c_end)  # R may run past the last stage (whole band launches)
         if G >= 0 and c0 <= G + 1:
             plan = hip.SweptPlan(state=t.ring.ptr, nx=t.nx, ny=t.ny, b=cls.b, ops=cls.ops, its=cls.its, T=T,
                                  threads=sw.threads, store_all=0, up=f["UpPyramid"], ybridge=f["YBridge"],
@@ -1110,8 +1121,8 @@ class Engine:
         if method not in ("swept", "standard", "band"):
             raise ValueError(f"unknown method {method!r} (swept, standard, band or auto)")
         if method == "band" and not cls.band_multi_ok:
-            raise ValueError(f"method='band' needs a plugin band sweep of one step per launch; class {cls.key} "
-                             "has none")
+            raise ValueError(f"method='band' needs a point update or a plugin band sweep of one step per "
+                             f"launch (kernels.band_multi_capable); class {cls.key} has neither")
         t.method = method
         if method in ("standard", "band"):
             t.times = schedule.standard_times(N, b, cls.ops, cls.its)

@@ -193,9 +193,18 @@ def band_capable(user_path, nv, ops, its):
 
 def band_multi_capable(user_path, nv, ops, its):
     """Whether problems of this plugin can run on the instance-table band
-    kernel (BandMulti): a plugin band sweep of one whole step per launch, so
-    that a problem's ring holds s + 1 levels, as on the standard kernel."""
-    return its == 2 and has_band_sweep(user_path) and not has_point_update(user_path)
+    kernel (BandMulti): a point update (launches of band_levels levels), or
+    a plugin band sweep of one whole step per launch (s = 2)."""
+    if not band_capable(user_path, nv, ops, its):
+        return False
+    return has_point_update(user_path) or its == 2
+
+
+def band_multi_levels(user_path, its, real="double"):
+    """Levels per instance-table band launch: the point update's default
+    depth (band_levels), or one step for a plugin band sweep.  A problem on
+    the band method keeps that many levels plus one in its ring."""
+    return band_levels(its, None, real) if has_point_update(user_path) else its
 
 
 #: cells from which a plugin's own band sweep (Euler: one column per lane,
@@ -336,18 +345,17 @@ def band_multi_rows(spec, shapes, simds=BAND_SIMDS):
 
 def band_spec(user_path, nv, ops, its, real="double", levels=None, defines=(), multi=False):
     """The band module of a plugin; ``multi``: its instance-table form
-    (BandMulti, one wavefront per workgroup), for a plugin band sweep of one
-    step per launch."""
+    (BandMulti, one wavefront per workgroup)."""
     if not band_capable(user_path, nv, ops, its):
         raise ValueError(f"{os.path.basename(user_path)} (nv={nv}, k={ops}, s={its}) has no band form: a point "
                          "update (PS_HAS_POINT, one variable, k=1) or a band sweep (PS_HAS_BAND), s <= 2")
     if levels is None and has_band_sweep(user_path) and not has_point_update(user_path):
         levels = its  # one step per launch: the registers hold a few rows of every level
     if multi and not band_multi_capable(user_path, nv, ops, its):
-        raise ValueError(f"{os.path.basename(user_path)} has no instance-table band form: a band sweep "
-                         "(PS_HAS_BAND) of one whole step per launch (s = 2)")
-    if multi and levels is not None and levels != its:
-        raise ValueError(f"an instance-table band launch takes one step ({its} levels), not {levels}")
+        raise ValueError(f"{os.path.basename(user_path)} has no instance-table band form: a point update, or a "
+                         "band sweep (PS_HAS_BAND) of one whole step per launch (s = 2)")
+    if multi and levels is not None and not has_point_update(user_path) and levels != its:
+        raise ValueError(f"an instance-table band launch of a band sweep takes one step ({its} levels), not {levels}")
     spec = ModuleSpec("band", os.path.abspath(user_path), nv, ops, its, threads=64 if multi else 256, real=real,
                       multi=bool(multi), levels=band_levels(its, levels, real), defines=tuple(defines))
     if band_out_columns(spec) < 8:

@@ -281,9 +281,10 @@ def test_tuner_records_the_fastest_candidate_and_results_do_not_depend_on_it(dev
             out.append(p)
         return out
 
-    cands = [{"b": 16}, {"b": 32}, {"b": 32, "slots": 4}, {"b": 32, "method": "standard"}]
+    cands = [{"b": 16}, {"b": 32}, {"b": 32, "slots": 4}, {"b": 32, "method": "standard"},
+             {"b": 32, "method": "band"}]
     rec = tu.tune("heat", "fp64", 1, make, candidates=cands)
-    assert rec["b"] in (16, 32) and rec["method"] in ("swept", "standard")
+    assert rec["b"] in (16, 32) and rec["method"] in ("swept", "standard", "band")
     with open(path) as f:
         stored = json.load(f)["heat/fp64/its1"]
     assert stored["b"] == rec["b"] and len(stored["trials"]) == len(cands)
@@ -304,7 +305,7 @@ def test_tune_classes_feeds_the_default_engine(device, tmp_path):
     recs = Tuner(path=path).tune_classes([p for p, _ in mix], sample=3, steps=6)
     assert set(recs) == {"euler/fp64/its2", "heat/fp64/its1", "heat/fp64/its2"}
     for r in recs.values():
-        assert r["method"] in ("swept", "standard") and r["workgroups_per_cu"] >= 1
+        assert r["method"] in ("swept", "standard", "band") and r["workgroups_per_cu"] >= 1
         assert len(r["trials"]) >= 3
     for p, _ in mix:
         p.method = "auto"
@@ -316,7 +317,7 @@ def test_tune_classes_feeds_the_default_engine(device, tmp_path):
         assert np.array_equal(t.result, oracle_stages(step, p.ic, p.its, p.its * steps))
 
 
-@pytest.mark.parametrize("method", ["swept", "standard"])
+@pytest.mark.parametrize("method", ["swept", "standard", "band"])
 def test_solo_path_equals_batched_rounds(device, method):
     """A problem running alone uses the plain kernels (swept cycles, then the
     exact-step tail); the same problem through batched rounds, and the
@@ -358,7 +359,14 @@ def _band_mix():
         out[i][0].method = "band"
     out[4][0].method = "standard"
     out[5][0].method = "swept"
-    return out + [_heat_problem(64, 64, 30)]
+    # heat on the band method: launches of 12 levels, the last one shorter
+    # (30 forward-Euler stages: 12 + 12 + 6; RK2 17 steps: 12 + 12 + 10;
+    # 5 stages: one short launch), beside a heat problem on the standard method
+    heat = [_heat_problem(64, 64, 30), _heat_problem(64, 64, 30, kx=2, amp=0.8),
+            _heat_problem(128, 64, 17, fe=False, ky=1), _heat_problem(64, 96, 5, kx=3)]
+    for p, _ in heat[1:]:
+        p.method = "band"
+    return out + heat
 
 
 def _check_oracle(tickets, mix, tag=""):
@@ -426,13 +434,15 @@ def test_band_method_graph_replay_equals_eager(device, rounds):
         assert np.array_equal(x.result, y.result)
 
 
-def test_band_method_rejected_without_a_band_sweep(device):
-    """Heat has a point update but no band sweep of one step per launch:
-    method="band" is refused at admission with a clear error."""
+def test_band_method_rejected_without_a_band_form(device, monkeypatch):
+    """A class without an instance-table band form: method="band" is
+    refused at admission with a clear error."""
+    from pysweep_amd import batch, kernels
     from pysweep_amd.batch import Engine
-    p, _ = _heat_problem(64, 64, 4)
+    monkeypatch.setattr(kernels, "band_multi_capable", lambda *a: False)
+    p, _ = _heat_problem(64, 64, 4, b=8)  # a class of its own (b=8)
     p.method = "band"
-    eng = Engine()
+    eng = Engine(tuner=None)
     eng.submit(p, arrival=0.0)
     with pytest.raises(ValueError, match="band sweep"):
         eng.run()
@@ -493,3 +503,35 @@ def test_band_method_under_selection_policies(device, policy, capacity):
     tickets = [eng.submit(p, arrival=0.003 * i) for i, (p, _) in enumerate(mix)]
     eng.run()
     _check_oracle(tickets, mix, f"{policy} {capacity}")
+
+
+def test_heat_band_method_hands_over_with_a_short_last_launch(device):
+    """A heat problem on the band method (launches of 12 stages) runs alone,
+    then beside another, then alone again; host steps of whole launches may
+    ask for more rounds than its stages left, and the one-problem path stops
+    at its last stage: results bitwise the oracle's."""
+    from pysweep_amd.batch import Engine
+    a, sa = _heat_problem(64, 64, 41, kx=2)  # 41 stages: 3 launches of 12 and one of 5
+    b, sb = _heat_problem(64, 128, 20, fe=False)  # RK2: 40 stages, 12 + 12 + 12 + 4
+    a.method = b.method = "band"
+    eng = Engine(max_rounds_ahead=5)
+    ta = eng.submit(a, arrival=0.0)
+    eng.step()
+    assert ta.c == 13, ta.c  # 5 rounds asked, one whole launch run
+    tb = eng.submit(b, arrival=0.0)
+    eng.run()
+    _check_oracle([ta, tb], [(a, sa), (b, sb)])
+
+
+def test_heat_band_method_fp32_equals_standard(device):
+    """FP32-state heat problems on the band method in a mix: each result
+    equals the same problem run alone on the standard method, bitwise."""
+    import dataclasses
+    mix = _band_mix()
+    f32 = [_heat_problem(64, 64, 25, kx=2, precision="fp32"), _heat_problem(128, 64, 14, precision="fp32")]
+    for p, _ in f32:
+        p.method = "band"
+    tickets, _ = _run([p for p, _ in mix + f32])
+    for i, (p, _) in enumerate(f32):
+        alone, _ = _run([dataclasses.replace(p, method="standard")], solo=False)
+        assert np.array_equal(alone[0].result, tickets[len(mix) + i].result), i

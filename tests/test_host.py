@@ -607,15 +607,27 @@ def test_band_multi_rows_and_whole_step_rounds():
     few = kernels.band_multi_rows(spec, [(1024, 1024)])[0]
     many = kernels.band_multi_rows(spec, [(1024, 1024)] * 16)[0]
     assert many > few
-    with pytest.raises(ValueError, match="instance-table band"):
-        kernels.band_spec(eq_path("heat.cu"), 1, 1, 2, multi=True)
-    assert not kernels.band_multi_capable(eq_path("heat.cu"), 1, 1, 2)
+    with pytest.raises(ValueError, match="band form"):
+        kernels.band_spec(eq_path("example.cu"), 1, 1, 1, multi=True)
+    with pytest.raises(ValueError, match="one step"):
+        kernels.band_spec(eq_path("euler.cu"), 4, 2, 2, levels=4, multi=True)
+    assert not kernels.band_multi_capable(eq_path("euler.cu"), 4, 2, 1)  # a band sweep takes RK2 steps
     assert kernels.band_multi_capable(eq_path("euler_fp32.cu"), 4, 2, 2)
-    band = SimpleNamespace(method="band", cls=SimpleNamespace(its=2))
-    std = SimpleNamespace(method="standard", cls=SimpleNamespace(its=2))
+    assert kernels.band_multi_capable(eq_path("heat.cu"), 1, 1, 1) and kernels.band_multi_capable(eq_path("heat.cu"), 1, 1, 2)
+    assert kernels.band_multi_levels(eq_path("heat.cu"), 1) == 12 and kernels.band_multi_levels(eq_path("euler.cu"), 2) == 2
+    assert kernels.band_multi_levels(eq_path("heat_fp32.cu"), 1, "float") == 8
+    heat = kernels.band_spec(eq_path("heat.cu"), 1, 1, 1, multi=True)
+    assert heat.levels == 12 and heat.threads == 64
+    assert all(r >= 1 for r in kernels.band_multi_rows(heat, [(64, 64), (1024, 96), (8, 200)]))
+    band = SimpleNamespace(method="band", cls=SimpleNamespace(its=2, band_h=2))
+    std = SimpleNamespace(method="standard", cls=SimpleNamespace(its=2, band_h=2))
+    hband = SimpleNamespace(method="band", cls=SimpleNamespace(its=1, band_h=12))
+    h8 = SimpleNamespace(method="band", cls=SimpleNamespace(its=1, band_h=8))
     assert batch.Engine._band_rounds([std], 3) == 3
     assert batch.Engine._band_rounds([std, band], 3) == 4
     assert batch.Engine._band_rounds([band], 6) == 6
+    assert batch.Engine._band_rounds([band, hband], 5) == 12
+    assert batch.Engine._band_rounds([hband, h8], 5) == 24  # whole launches of both
     assert any(sp.kind == "band" and sp.multi for sp in batch.shipped_class_specs())
 
 
